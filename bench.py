@@ -1,0 +1,131 @@
+#!/usr/bin/env python3
+"""Benchmark of the FP64 finite-volume time step on one MI355X.
+
+Workload (the headline configuration): single-phase 1D, 2^22 cells on [0,1],
+65 seeded piecewise-constant states (A = I + 0.05 N(0,1), u ~ U(-1/2,1/2)^3,
+S = 1e-3, rho = rho0/det A) separated by 64 seeded interfaces, Barton2009
+copper constants, CFL 0.6, Rusanov flux.  A "step" is one time step over all
+cells (one launch of the fused step kernel).  The state is resident in HBM
+before the timed region; HIP events on the solver's stream bracket exactly
+the K timed launches.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 cell-updates/s on 1 GPU (% HBM peak); batched vs plain; fwd+adj/fwd"
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
+BYTES_PER_CELL_1PH = 2 * 14 * 8  # read + write the 14-component FP64 state
+
+
+def headline_problem(nx, seed=2):
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    from hyperelasticsolver_amd.initial import prim2cons, random_interfaces, random_phase_prims
+    eos = Barton2009()
+    rng = np.random.default_rng(seed)
+    P = random_phase_prims(rng, eos, 65)
+    Qs = prim2cons(eos, P)
+    ifc = random_interfaces(rng, 64)
+    return eos, Qs, ifc
+
+
+def time_solver(eos, Qs, ifc, nx, flux, steps, warmup):
+    from hyperelasticsolver_amd.solver import Solver
+    s = Solver(eos, nx, flux=flux, cfl=0.6)
+    s.set_piecewise(Qs, ifc)
+    if warmup:
+        s.time_steps(warmup)
+    ms = s.time_steps(steps)
+    Q = s.get_state()
+    s.close()
+    if not np.all(np.isfinite(Q)):
+        raise RuntimeError("non-finite state after the timed steps")
+    return ms
+
+
+def cpu_baseline(eos, Qs, ifc, flux_kind, threads):
+    """The oracle (C++ restatement of the reference algorithm) on host cores,
+    on a bounded sample of the same workload: 2^16 cells, 8 steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind
+    ora = oracle_bind.load()
+    nx = 1 << 16
+    steps = 8
+    x = (np.arange(nx) + 0.5) / nx
+    Q0 = np.asfortranarray(Qs[:, np.searchsorted(ifc, x, side="left")])
+    t0 = time.perf_counter()
+    _, _, s, _ = ora.run(1, eos.params(), flux_kind, Q0, 0.6, 1.0 / nx, steps=steps, threads=threads)
+    el = time.perf_counter() - t0
+    return {"value": nx * s / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{nx} cells x {s} steps of the same workload ({'Rusanov' if flux_kind == 0 else 'HLL'}),"
+                      f" OpenMP over cells, {el:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=1 << 22)
+    ap.add_argument("--flux", default="rusanov", choices=["rusanov", "hll"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the HLL line")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU slab "
+                                   "decomposition is a design (see DESIGN.md), not measured",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from hyperelasticsolver_amd.solver import device_info
+    nx = args.nx
+    eos, Qs, ifc = headline_problem(nx)
+    ms = time_solver(eos, Qs, ifc, nx, args.flux, args.steps, args.warmup)
+    ms_step = ms / args.steps
+    value = nx * args.steps / (ms / 1e3)
+    achieved = BYTES_PER_CELL_1PH * nx / (ms_step / 1e3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"single-phase 1D Riemann, {nx} cells, 64 seeded interfaces, "
+                               f"{args.flux}, CFL 0.6, Barton2009 copper",
+                   "cells": nx, "flux": args.flux, "interfaces": 64},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_step<1,{0 if args.flux == 'rusanov' else 1}>",
+                     "bytes_per_cell_update": BYTES_PER_CELL_1PH},
+        "device": device_info(),
+    }
+    if not args.no_extra:
+        other = "hll" if args.flux == "rusanov" else "rusanov"
+        ms2 = time_solver(eos, Qs, ifc, nx, other, args.steps, args.warmup)
+        v2 = nx * args.steps / (ms2 / 1e3)
+        out[other] = {"value": v2, "ms_per_step": ms2 / args.steps,
+                      "hbm_frac": BYTES_PER_CELL_1PH * v2 / 1e9 / HBM_PEAK_GBS}
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+        out["cpu_baseline"] = cpu_baseline(eos, Qs, ifc, 0 if args.flux == "rusanov" else 1, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,517 @@
+// hes_device.hpp -- gfx950 device math for the Barton-2009 hyperelastic
+// finite-volume step (one phase block at a time).
+//
+// Everything here is FP64 and lives in registers.  The reference evaluates the
+// stress and the acoustic tensor with nested ForwardDiff duals
+// (EquationsOfState.jl:179-190, 223-246); this file uses closed forms of the
+// same derivatives instead:
+//
+//   energy  e(S,G) = k0/(2a^2)(s_a-1)^2 + cv t0 s_g (x-1) + b0^2/2 s_b (I1^2/3 - I2)
+//           with s_k = I3^(k/2), x = exp(S/cv)              (EquationsOfState.jl:118-137)
+//   stress  sigma = -2 rho H,  H = kappa (G^2 - I1/3 G) + c3 I,
+//           kappa = b0^2 s_b / 2,
+//           c3 = k0/(2a) (s_a-1) s_a + (g/2) E_th + (b/2) W  (= I3 de/dI3)
+//           (the product G * dedG of EquationsOfState.jl:183-188 for symmetric G)
+//   acoustic Omega_ij = (1/rho) d sigma(n,i) / dA along dA = e_j (A^T n)^T, giving
+//           Omega_ij = 2 n_j (H n)_i - 2 (n^T dH_j)_i with dG_j = -(G e_j n^T + n e_j^T G)
+//           (EquationsOfState.jl:226-243 contracted with n)
+//
+// The entropy never has to be formed on the step path: S = cv log(x) and the
+// stress and acoustic tensor need only x = exp(S/cv), which cons->prim yields
+// directly as the clamped argument of the logarithm (EquationsOfState.jl:150-155).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace hes {
+
+// Material constants of Barton2009 (EquationsOfState.jl:71-116), with the
+// derived quantities the kernels use precomputed on the host.
+struct MatDev {
+  double rho0, irho0, cv, t0, b0sq, k0, alpha, beta, gamma;
+  double hk0a2;   // k0 / (2 alpha^2)      : cold-energy prefactor
+  double k0_2a;   // k0 / (2 alpha)        : in c3
+  double k0_2;    // k0 / 2                : in dc3
+  double cvt0;    // cv * t0
+  double icvt0;   // 1 / (cv t0)
+  double hb0sq;   // b0^2 / 2
+  double half_a, half_b, half_g;  // alpha/2, beta/2, gamma/2
+  double g2_2;    // gamma^2 / 2
+  int fast;       // alpha == 1, beta == 3, gamma == 2 (no pow needed)
+  int pad;
+};
+
+// State of one phase derived from its conserved block.
+struct Thermo {
+  double rho;            // true density
+  double den;            // frac * rho
+  double u[3];           // velocity
+  double E;              // specific total energy
+  double A[9];           // distortion tensor, column-major
+  double G11, G22, G33, G12, G13, G23;  // Finger tensor (A A^T)^-1
+  double i1, i2, i3;     // invariants of G
+  double sa, sb, sg;     // I3^(alpha/2), I3^(beta/2), I3^(gamma/2)
+  double W;              // shear energy
+  double Eth;            // thermal energy cv t0 s_g (x - 1)
+  double x;              // exp(S/cv), clamped at 1e-6 as the entropy law does
+  int bad;               // non-positive det(rho A) or non-finite state
+};
+
+__device__ __forceinline__ double det3cm(const double* a) {
+  // column-major a(i,j) = a[i + 3j]
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[3] * (a[1] * a[8] - a[7] * a[2]) +
+         a[6] * (a[1] * a[5] - a[4] * a[2]);
+}
+
+// Finger tensor, invariants and energy split of a phase.
+// q points at [a r, a r u(3), a r E, a r A(9)] (the phase block without a).
+// HyperelasticityMPh.jl:106-124 (cons2prim) + Strains.jl:26-52 + EquationsOfState.jl:139-156.
+template <bool ONE_FRAC>
+__device__ __forceinline__ void thermo(const MatDev& m, double frac, const double* q, Thermo& t) {
+  const double* P = q + 5;
+  double detP = det3cm(P);
+  double rho2;
+  if (ONE_FRAC) {
+    rho2 = detP * m.irho0;
+  } else {
+    double ifr = 1.0 / frac;
+    rho2 = detP * (ifr * ifr * ifr) * m.irho0;
+  }
+  t.bad = !(detP > 0.0);
+  t.rho = sqrt(rho2);
+  t.den = ONE_FRAC ? t.rho : frac * t.rho;
+  double iden = 1.0 / t.den;
+  t.u[0] = q[1] * iden;
+  t.u[1] = q[2] * iden;
+  t.u[2] = q[3] * iden;
+  t.E = q[4] * iden;
+  double e = t.E - 0.5 * (t.u[0] * t.u[0] + t.u[1] * t.u[1] + t.u[2] * t.u[2]);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) t.A[k] = P[k] * iden;
+  const double* A = t.A;
+  // B = A A^T (symmetric)
+  double B11 = A[0] * A[0] + A[3] * A[3] + A[6] * A[6];
+  double B22 = A[1] * A[1] + A[4] * A[4] + A[7] * A[7];
+  double B33 = A[2] * A[2] + A[5] * A[5] + A[8] * A[8];
+  double B12 = A[0] * A[1] + A[3] * A[4] + A[6] * A[7];
+  double B13 = A[0] * A[2] + A[3] * A[5] + A[6] * A[8];
+  double B23 = A[1] * A[2] + A[4] * A[5] + A[7] * A[8];
+  double C11 = B22 * B33 - B23 * B23;
+  double C12 = B13 * B23 - B12 * B33;
+  double C13 = B12 * B23 - B13 * B22;
+  double detB = B11 * C11 + B12 * C12 + B13 * C13;
+  double idetB = 1.0 / detB;
+  t.G11 = C11 * idetB;
+  t.G22 = (B11 * B33 - B13 * B13) * idetB;
+  t.G33 = (B11 * B22 - B12 * B12) * idetB;
+  t.G12 = C12 * idetB;
+  t.G13 = C13 * idetB;
+  t.G23 = (B12 * B13 - B11 * B23) * idetB;
+  t.i1 = t.G11 + t.G22 + t.G33;
+  double trG2 = t.G11 * t.G11 + t.G22 * t.G22 + t.G33 * t.G33 +
+                2.0 * (t.G12 * t.G12 + t.G13 * t.G13 + t.G23 * t.G23);
+  t.i2 = 0.5 * (t.i1 * t.i1 - trG2);
+  t.i3 = idetB;
+  double ith;  // 1 / (cv t0 s_g)
+  if (m.fast) {
+    // det A = rho0 / rho, so I3^(1/2) = 1/det A = rho / rho0
+    t.sa = t.rho * m.irho0;
+    t.sb = t.i3 * t.sa;
+    t.sg = t.i3;
+    ith = detB * m.icvt0;
+  } else {
+    double li3 = log(t.i3);
+    t.sa = exp(m.half_a * li3);
+    t.sb = exp(m.half_b * li3);
+    t.sg = exp(m.half_g * li3);
+    ith = 1.0 / (m.cvt0 * t.sg);
+  }
+  t.W = m.hb0sq * t.sb * (t.i1 * t.i1 * (1.0 / 3.0) - t.i2);
+  double sam1 = t.sa - 1.0;
+  double Uc = m.hk0a2 * sam1 * sam1;
+  double x = (e - t.W - Uc) * ith + 1.0;
+  if (x < 1e-6) x = 1e-6;
+  t.x = x;
+  t.Eth = m.cvt0 * t.sg * (x - 1.0);
+  t.bad |= !(e == e) | !(t.rho < 1e300);
+}
+
+// Entropy S = cv log x (EquationsOfState.jl:155).
+__device__ __forceinline__ double entropy_of(const MatDev& m, const Thermo& t) {
+  return m.cv * log(t.x);
+}
+
+// c3 = I3 de/dI3 at fixed entropy.
+__device__ __forceinline__ double c3_of(const MatDev& m, const Thermo& t) {
+  return m.k0_2a * (t.sa - 1.0) * t.sa + m.half_g * t.Eth + m.half_b * t.W;
+}
+
+// First row of the Cauchy stress sigma (= first column; sigma is symmetric).
+__device__ __forceinline__ void stress_row1(const MatDev& m, const Thermo& t, double s[3]) {
+  double G2_11 = t.G11 * t.G11 + t.G12 * t.G12 + t.G13 * t.G13;
+  double G2_12 = t.G11 * t.G12 + t.G12 * t.G22 + t.G13 * t.G23;
+  double G2_13 = t.G11 * t.G13 + t.G12 * t.G23 + t.G13 * t.G33;
+  double kap = m.hb0sq * t.sb;
+  double c3 = c3_of(m, t);
+  double i13 = t.i1 * (1.0 / 3.0);
+  double m2r = -2.0 * t.rho;
+  s[0] = m2r * (kap * (G2_11 - i13 * t.G11) + c3);
+  s[1] = m2r * (kap * (G2_12 - i13 * t.G12));
+  s[2] = m2r * (kap * (G2_13 - i13 * t.G13));
+}
+
+// Full stress, column-major 9-vector.
+__device__ __forceinline__ void stress_full(const MatDev& m, const Thermo& t, double s[9]) {
+  double G[9] = {t.G11, t.G12, t.G13, t.G12, t.G22, t.G23, t.G13, t.G23, t.G33};
+  double kap = m.hb0sq * t.sb;
+  double c3 = c3_of(m, t);
+  double i13 = t.i1 * (1.0 / 3.0);
+  double m2r = -2.0 * t.rho;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double g2 = G[i] * G[3 * j] + G[i + 3] * G[3 * j + 1] + G[i + 6] * G[3 * j + 2];
+      double h = kap * (g2 - i13 * G[i + 3 * j]) + (i == j ? c3 : 0.0);
+      s[i + 3 * j] = m2r * h;
+    }
+}
+
+// Temperature de/dS = t0 s_g x (HyperelasticityMPh.jl:212).
+__device__ __forceinline__ double temperature_of(const MatDev& m, const Thermo& t) {
+  return m.t0 * t.sg * t.x;
+}
+
+// Acoustic tensor for n = e1 (the 1D solver's normal), symmetric: returns
+// O11 O12 O13 O22 O23 O33.
+__device__ __forceinline__ void acoustic_e1(const MatDev& m, const Thermo& t, double O[6]) {
+  const double G11 = t.G11, G22 = t.G22, G33 = t.G33, G12 = t.G12, G13 = t.G13, G23 = t.G23;
+  double G2_11 = G11 * G11 + G12 * G12 + G13 * G13;
+  double G2_12 = G11 * G12 + G12 * G22 + G13 * G23;
+  double G2_13 = G11 * G13 + G12 * G23 + G13 * G33;
+  double G2_22 = G12 * G12 + G22 * G22 + G23 * G23;
+  double G2_23 = G12 * G13 + G22 * G23 + G23 * G33;
+  double G2_33 = G13 * G13 + G23 * G23 + G33 * G33;
+  double kap = m.hb0sq * t.sb;
+  double c3 = c3_of(m, t);
+  double i1 = t.i1, i13 = i1 * (1.0 / 3.0);
+  double H11 = kap * (G2_11 - i13 * G11) + c3;
+  double H12 = kap * (G2_12 - i13 * G12);
+  double H13 = kap * (G2_13 - i13 * G13);
+  double hs = m.hb0sq * t.sb;  // dW = -beta W n_j + hs ((2/3) i1 di1 - di2)
+  // --- direction j = 1 (n_j = 1)
+  {
+    double di1 = -2.0 * G11;
+    double di2 = i1 * di1 + 2.0 * G2_11;
+    double dkap = -m.beta * kap;
+    double dW = -m.beta * t.W + hs * ((2.0 / 3.0) * i1 * di1 - di2);
+    double dc3 = -m.k0_2 * (2.0 * t.sa - 1.0) * t.sa - m.g2_2 * t.Eth + m.half_b * dW;
+    double di13 = di1 * (1.0 / 3.0);
+    // i = 1
+    double dG = -2.0 * G11;
+    double dG2 = -2.0 * (G2_11 + G11 * G11);
+    double dH = dkap * (G2_11 - i13 * G11) + kap * (dG2 - di13 * G11 - i13 * dG) + dc3;
+    O[0] = 2.0 * H11 - 2.0 * dH;
+    // i = 2
+    dG = -G12;
+    dG2 = -(2.0 * G11 * G12 + G2_12);
+    dH = dkap * (G2_12 - i13 * G12) + kap * (dG2 - di13 * G12 - i13 * dG);
+    O[1] = 2.0 * H12 - 2.0 * dH;
+    // i = 3
+    dG = -G13;
+    dG2 = -(2.0 * G11 * G13 + G2_13);
+    dH = dkap * (G2_13 - i13 * G13) + kap * (dG2 - di13 * G13 - i13 * dG);
+    O[2] = 2.0 * H13 - 2.0 * dH;
+  }
+  // --- direction j = 2 (n_j = 0): i = 2, 3
+  {
+    double di13 = -2.0 * G12 * (1.0 / 3.0);
+    double dG = -G22;
+    double dG2 = -(G11 * G22 + G12 * G12 + G2_22);
+    double dH = kap * (dG2 - di13 * G12 - i13 * dG);
+    O[3] = -2.0 * dH;
+    dG = -G23;
+    dG2 = -(G11 * G23 + G12 * G13 + G2_23);
+    dH = kap * (dG2 - di13 * G13 - i13 * dG);
+    O[4] = -2.0 * dH;
+  }
+  // --- direction j = 3: i = 3
+  {
+    double di13 = -2.0 * G13 * (1.0 / 3.0);
+    double dG = -G33;
+    double dG2 = -(G11 * G33 + G13 * G13 + G2_33);
+    double dH = kap * (dG2 - di13 * G13 - i13 * dG);
+    O[5] = -2.0 * dH;
+  }
+}
+
+// Acoustic tensor for a general direction n, column-major 3x3.
+//   fixed_e == false: acoustic(eos, ent, F, n) (EquationsOfState.jl:223-246):
+//                     entropy held, rho = rho0/det A varies with A.
+//   fixed_e == true : acoustic(eos, P, n) (EquationsOfState.jl:192-221): the
+//                     stress is stress(eos, den, e_int, F) with den and e_int
+//                     held, so the entropy follows A (unless the entropy law's
+//                     clamp is active, `clamped`) and there is no density term.
+__device__ __forceinline__ void acoustic_n(const MatDev& m, const Thermo& t, const double n[3],
+                                           double O[9], bool fixed_e = false,
+                                           bool clamped = false) {
+  double G[9] = {t.G11, t.G12, t.G13, t.G12, t.G22, t.G23, t.G13, t.G23, t.G33};
+  double G2[9];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      G2[i + 3 * j] = G[i] * G[3 * j] + G[i + 3] * G[3 * j + 1] + G[i + 6] * G[3 * j + 2];
+  double gn[3], g2n[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    gn[i] = G[i] * n[0] + G[i + 3] * n[1] + G[i + 6] * n[2];
+    g2n[i] = G2[i] * n[0] + G2[i + 3] * n[1] + G2[i + 6] * n[2];
+  }
+  double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  double nGn = n[0] * gn[0] + n[1] * gn[1] + n[2] * gn[2];
+  double kap = m.hb0sq * t.sb;
+  double c3 = c3_of(m, t);
+  double i1 = t.i1, i13 = i1 * (1.0 / 3.0);
+  double Hn[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) Hn[i] = kap * (g2n[i] - i13 * gn[i]) + c3 * n[i];
+  double hs = m.hb0sq * t.sb;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double nj = n[j];
+    double di1 = -2.0 * gn[j];
+    double di2 = i1 * di1 + 2.0 * g2n[j];
+    double dkap = -m.beta * kap * nj;
+    double dW = -m.beta * t.W * nj + hs * ((2.0 / 3.0) * i1 * di1 - di2);
+    double dsa = -m.alpha * t.sa * nj;
+    double dEth;
+    if (fixed_e && !clamped) {
+      double dUc = 2.0 * m.hk0a2 * (t.sa - 1.0) * dsa;
+      dEth = -(dW + dUc);
+    } else {
+      dEth = -m.gamma * t.Eth * nj;
+    }
+    double dc3 = m.k0_2a * (2.0 * t.sa - 1.0) * dsa + m.half_g * dEth + m.half_b * dW;
+    double di13 = di1 * (1.0 / 3.0);
+    double rterm = fixed_e ? 0.0 : 2.0 * nj;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double ndG = -(gn[j] * n[i] + nn * G[i + 3 * j]);
+      double ndG2 = -(g2n[j] * n[i] + nGn * G[i + 3 * j] + gn[j] * gn[i] + nn * G2[i + 3 * j]);
+      double ndH = dkap * (g2n[i] - i13 * gn[i]) + kap * (ndG2 - di13 * gn[i] - i13 * ndG) +
+                   dc3 * n[i];
+      O[i + 3 * j] = rterm * Hn[i] - 2.0 * ndH;
+    }
+  }
+}
+
+// Largest root of t^3 - c2 t^2 + c1 t - c0 (all roots real) by Newton from an
+// upper bound; the iteration decreases monotonically onto the root.
+__device__ __forceinline__ double cubic_top_root(double c2, double c1, double c0, double ub,
+                                                 double lb) {
+  double t = ub;
+#pragma unroll 1
+  for (int it = 0; it < 60; ++it) {
+    double f = ((t - c2) * t + c1) * t - c0;
+    double fp = (3.0 * t - 2.0 * c2) * t + c1;
+    if (!(fp > 0.0)) break;
+    double tn = t - f / fp;
+    if (!(tn < t)) break;
+    t = tn < lb ? lb : tn;
+  }
+  return t;
+}
+
+// Smallest root, Newton from a lower bound (increases monotonically).
+__device__ __forceinline__ double cubic_bottom_root(double c2, double c1, double c0, double lb,
+                                                    double ub) {
+  double t = lb;
+#pragma unroll 1
+  for (int it = 0; it < 60; ++it) {
+    double f = ((t - c2) * t + c1) * t - c0;
+    double fp = (3.0 * t - 2.0 * c2) * t + c1;
+    if (!(fp > 0.0)) break;
+    double tn = t - f / fp;
+    if (!(tn > t)) break;
+    t = tn > ub ? ub : tn;
+  }
+  return t;
+}
+
+// Spectral radius of a symmetric 3x3 given as O11 O12 O13 O22 O23 O33.
+// HyperelasticityMPh.jl:263 uses sqrt(|eig|); only the largest modulus is
+// ever consumed (main.jl:210, NumFluxes.jl:90-91).
+__device__ __forceinline__ double spectral_radius_sym(const double O[6]) {
+  double a = O[0], b = O[3], c = O[5], d = O[1], e = O[2], f = O[4];
+  double c2 = a + b + c;
+  double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
+  double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
+  double r1 = fabs(d) + fabs(e), r2 = fabs(d) + fabs(f), r3 = fabs(e) + fabs(f);
+  double ub = fmax(a + r1, fmax(b + r2, c + r3));
+  double lb = fmin(a - r1, fmin(b - r2, c - r3));
+  double top = cubic_top_root(c2, c1, c0, ub, lb);
+  double rad = fabs(top);
+  if (lb < -rad) {
+    double bot = cubic_bottom_root(c2, c1, c0, lb, ub);
+    rad = fmax(rad, fabs(bot));
+  }
+  return rad;
+}
+
+// All three eigenvalues of a symmetric 3x3 (descending): the largest by Newton,
+// the other two from the deflated quadratic.
+__device__ __forceinline__ void eig3_sym(const double O[6], double w[3]) {
+  double a = O[0], b = O[3], c = O[5], d = O[1], e = O[2], f = O[4];
+  double c2 = a + b + c;
+  double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
+  double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
+  double r1 = fabs(d) + fabs(e), r2 = fabs(d) + fabs(f), r3 = fabs(e) + fabs(f);
+  double ub = fmax(a + r1, fmax(b + r2, c + r3));
+  double lb = fmin(a - r1, fmin(b - r2, c - r3));
+  double l1 = cubic_top_root(c2, c1, c0, ub, lb);
+  double l3 = cubic_bottom_root(c2, c1, c0, lb, ub);
+  double l2 = c2 - l1 - l3;
+  w[0] = l1;
+  w[1] = l2;
+  w[2] = l3;
+}
+
+// Sound speed c_max = sqrt(rho(Omega)) of a phase for n = e1.
+__device__ __forceinline__ double sound_speed(const MatDev& m, const Thermo& t) {
+  double O[6];
+  acoustic_e1(m, t, O);
+  return sqrt(spectral_radius_sym(O));
+}
+
+// Physical flux of a phase (HyperelasticityMPh.jl:146-175) into f[14]
+// = flux block without the volume-fraction entry (which is 0).
+__device__ __forceinline__ void phase_flux(const MatDev& m, double frac, const Thermo& t,
+                                           double f[14]) {
+  double s[3];
+  stress_row1(m, t, s);
+  double st0 = frac * s[0], st1 = frac * s[1], st2 = frac * s[2];
+  const double* u = t.u;
+  double du1 = t.den * u[0];
+  f[0] = du1;
+  f[1] = du1 * u[0] - st0;
+  f[2] = du1 * u[1] - st1;
+  f[3] = du1 * u[2] - st2;
+  f[4] = du1 * t.E - (u[0] * st0 + u[1] * st1 + u[2] * st2);
+  const double* A = t.A;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    f[5 + 3 * c] = 0.0;  // row 1: u1 A_1c - u1 A_1c
+    f[6 + 3 * c] = t.den * (u[0] * A[1 + 3 * c] - u[1] * A[3 * c]);
+    f[7 + 3 * c] = t.den * (u[0] * A[2 + 3 * c] - u[2] * A[3 * c]);
+  }
+}
+
+// Energy e(S, G) of a phase given A and x = exp(S/cv) (prim->cons).
+__device__ __forceinline__ double energy_from_A(const MatDev& m, const double A[9], double x) {
+  double B11 = A[0] * A[0] + A[3] * A[3] + A[6] * A[6];
+  double B22 = A[1] * A[1] + A[4] * A[4] + A[7] * A[7];
+  double B33 = A[2] * A[2] + A[5] * A[5] + A[8] * A[8];
+  double B12 = A[0] * A[1] + A[3] * A[4] + A[6] * A[7];
+  double B13 = A[0] * A[2] + A[3] * A[5] + A[6] * A[8];
+  double B23 = A[1] * A[2] + A[4] * A[5] + A[7] * A[8];
+  double C11 = B22 * B33 - B23 * B23;
+  double C12 = B13 * B23 - B12 * B33;
+  double C13 = B12 * B23 - B13 * B22;
+  double detB = B11 * C11 + B12 * C12 + B13 * C13;
+  double idetB = 1.0 / detB;
+  double G11 = C11 * idetB, G22 = (B11 * B33 - B13 * B13) * idetB;
+  double G33 = (B11 * B22 - B12 * B12) * idetB;
+  double G12 = C12 * idetB, G13 = C13 * idetB, G23 = (B12 * B13 - B11 * B23) * idetB;
+  double i1 = G11 + G22 + G33;
+  double trG2 = G11 * G11 + G22 * G22 + G33 * G33 + 2.0 * (G12 * G12 + G13 * G13 + G23 * G23);
+  double i2 = 0.5 * (i1 * i1 - trG2);
+  double i3 = idetB;
+  double li3 = log(i3);
+  double sa = exp(m.half_a * li3), sb = exp(m.half_b * li3), sg = exp(m.half_g * li3);
+  double sam1 = sa - 1.0;
+  return m.hk0a2 * sam1 * sam1 + m.cvt0 * sg * (x - 1.0) +
+         m.hb0sq * sb * (i1 * i1 * (1.0 / 3.0) - i2);
+}
+
+// Invariants of a general (column-major) G and the energy/entropy laws as
+// functions of G (EquationsOfState.jl:118-156), for the point-wise API.
+__device__ __forceinline__ void invariants_gen(const double* g, double inv[3]) {
+  double tr = g[0] + g[4] + g[8];
+  double trg2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) trg2 += g[i + 3 * k] * g[k + 3 * i];
+  inv[0] = tr;
+  inv[1] = 0.5 * (tr * tr - trg2);
+  inv[2] = det3cm(g);
+}
+
+__device__ __forceinline__ double energy_G(const MatDev& m, double S, const double* g) {
+  double inv[3];
+  invariants_gen(g, inv);
+  double li3 = log(inv[2]);
+  double sa = exp(m.half_a * li3), sb = exp(m.half_b * li3), sg = exp(m.half_g * li3);
+  double sam1 = sa - 1.0;
+  return m.hk0a2 * sam1 * sam1 + m.cvt0 * sg * (exp(S / m.cv) - 1.0) +
+         m.hb0sq * sb * (inv[0] * inv[0] * (1.0 / 3.0) - inv[1]);
+}
+
+__device__ __forceinline__ double entropy_G(const MatDev& m, double e_int, const double* g) {
+  double inv[3];
+  invariants_gen(g, inv);
+  double li3 = log(inv[2]);
+  double sa = exp(m.half_a * li3), sb = exp(m.half_b * li3), sg = exp(m.half_g * li3);
+  double sam1 = sa - 1.0;
+  double S = e_int - m.hb0sq * sb * (inv[0] * inv[0] * (1.0 / 3.0) - inv[1]) -
+             m.hk0a2 * sam1 * sam1;
+  S = S / (m.cvt0 * sg) + 1.0;
+  if (S < 1e-6) S = 1e-6;
+  return log(S) * m.cv;
+}
+
+// Thermo state of a phase from a distortion A and x = exp(S/cv) (rho = rho0/det A),
+// for the point-wise stress/acoustic API.
+__device__ __forceinline__ void thermo_from_A(const MatDev& m, const double* A, double x,
+                                              Thermo& t) {
+  double detA = det3cm(A);
+  t.bad = !(detA > 0.0);
+  t.rho = m.rho0 / detA;
+  t.den = t.rho;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) t.A[k] = A[k];
+  t.u[0] = t.u[1] = t.u[2] = 0.0;
+  t.E = 0.0;
+  double B11 = A[0] * A[0] + A[3] * A[3] + A[6] * A[6];
+  double B22 = A[1] * A[1] + A[4] * A[4] + A[7] * A[7];
+  double B33 = A[2] * A[2] + A[5] * A[5] + A[8] * A[8];
+  double B12 = A[0] * A[1] + A[3] * A[4] + A[6] * A[7];
+  double B13 = A[0] * A[2] + A[3] * A[5] + A[6] * A[8];
+  double B23 = A[1] * A[2] + A[4] * A[5] + A[7] * A[8];
+  double C11 = B22 * B33 - B23 * B23;
+  double C12 = B13 * B23 - B12 * B33;
+  double C13 = B12 * B23 - B13 * B22;
+  double detB = B11 * C11 + B12 * C12 + B13 * C13;
+  double idetB = 1.0 / detB;
+  t.G11 = C11 * idetB;
+  t.G22 = (B11 * B33 - B13 * B13) * idetB;
+  t.G33 = (B11 * B22 - B12 * B12) * idetB;
+  t.G12 = C12 * idetB;
+  t.G13 = C13 * idetB;
+  t.G23 = (B12 * B13 - B11 * B23) * idetB;
+  t.i1 = t.G11 + t.G22 + t.G33;
+  double trG2 = t.G11 * t.G11 + t.G22 * t.G22 + t.G33 * t.G33 +
+                2.0 * (t.G12 * t.G12 + t.G13 * t.G13 + t.G23 * t.G23);
+  t.i2 = 0.5 * (t.i1 * t.i1 - trG2);
+  t.i3 = idetB;
+  double li3 = log(t.i3);
+  t.sa = exp(m.half_a * li3);
+  t.sb = exp(m.half_b * li3);
+  t.sg = exp(m.half_g * li3);
+  t.W = m.hb0sq * t.sb * (t.i1 * t.i1 * (1.0 / 3.0) - t.i2);
+  t.x = x;
+  t.Eth = m.cvt0 * t.sg * (x - 1.0);
+}
+
+}  // namespace hes

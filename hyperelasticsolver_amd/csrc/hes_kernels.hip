@@ -1,0 +1,1076 @@
+// hes_kernels.hip -- gfx950 kernels of the finite-volume time step and the
+// C-ABI of libhes.so (declared in include/hes.h).
+//
+// Time step (main.jl:202-241).  One launch per step:
+//   k_step<NPH,FLUX>  reads Q^n and the per-cell sound speeds c^n, takes
+//                     dt^n = cfl dx / max lambda^n from a ring slot written by
+//                     the previous launch, evaluates each cell's flux once and
+//                     each face once (main.jl evaluates every face twice),
+//                     updates the interior cells, then computes c^{n+1} of the
+//                     new state and max-reduces lambda^{n+1} for the next step.
+// State is structure-of-arrays FP64: component k of cell i at Q[k*pitch + i].
+// A 64-lane wave owns a tile of 62 cells plus one halo cell on each side, so
+// neighbour exchange is a lane shuffle and needs no LDS or barrier.
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+#include <string>
+#include <vector>
+#include <mutex>
+
+#include "hes_device.hpp"
+#include "hes_faces.hpp"
+#include "../../include/hes.h"
+
+using namespace hes;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e__ = (expr);                                                       \
+    if (e__ != hipSuccess) {                                                       \
+      g_err = std::string(#expr) + ": " + hipGetErrorString(e__);                  \
+      return HES_ERR_HIP;                                                          \
+    }                                                                              \
+  } while (0)
+
+constexpr int TILE = 62;      // owned cells per wave
+constexpr int BLOCK = 256;    // 4 waves
+
+MatDev make_mat(const double* p) {
+  MatDev m;
+  std::memset(&m, 0, sizeof(m));
+  m.rho0 = p[0];
+  m.irho0 = 1.0 / p[0];
+  m.cv = p[2];
+  m.t0 = p[3];
+  double b0 = p[4];
+  m.alpha = p[5];
+  m.beta = p[6];
+  m.gamma = p[7];
+  m.b0sq = b0 * b0;  // EquationsOfState.jl:111-112
+  m.k0 = p[1] * p[1] - (4.0 / 3.0) * (b0 * b0);
+  m.hk0a2 = 0.5 * m.k0 / (m.alpha * m.alpha);
+  m.k0_2a = m.k0 / (2.0 * m.alpha);
+  m.k0_2 = 0.5 * m.k0;
+  m.cvt0 = m.cv * m.t0;
+  m.icvt0 = 1.0 / m.cvt0;
+  m.hb0sq = 0.5 * m.b0sq;
+  m.half_a = 0.5 * m.alpha;
+  m.half_b = 0.5 * m.beta;
+  m.half_g = 0.5 * m.gamma;
+  m.g2_2 = 0.5 * m.gamma * m.gamma;
+  m.fast = (m.alpha == 1.0 && m.beta == 3.0 && m.gamma == 2.0) ? 1 : 0;
+  return m;
+}
+
+bool mat_ok(const double* p) {
+  for (int k = 0; k < 8; ++k)
+    if (!std::isfinite(p[k])) return false;
+  return p[0] > 0 && p[2] > 0 && p[3] > 0 && p[5] != 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Step kernel
+// ---------------------------------------------------------------------------
+struct StepArgs {
+  const double* __restrict__ Qin;
+  double* __restrict__ Qout;
+  const double* __restrict__ Cin;
+  double* __restrict__ Cout;
+  unsigned long long* slots;  // ring of 3: max lambda of state s lives in slots[s % 3]
+  double* tring;              // ring of 2: time of state s in tring[s & 1]
+  double* dt_hist;            // optional: dt of each executed step
+  int* status;                // [0] error bits, [1] executed steps
+  long long n, pitch;
+  double cfl, dx, t_end;
+  int step;                   // step index modulo 6 (ring phase)
+  int hist_idx;
+  MatDev mat[2];
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+__device__ __forceinline__ void block_max_to_slot(double v, unsigned long long* slot) {
+  __shared__ double red[BLOCK / 64];
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+#pragma unroll
+    for (int k = 1; k < BLOCK / 64; ++k) m = fmax(m, red[k]);
+    if (m > 0.0) {
+      unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+      unsigned long long cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (bits > cur) atomicMax(slot, bits);
+    }
+  }
+}
+
+template <int NPH, int FLUX>
+__global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
+  constexpr int NC = NComp<NPH>::value;
+  const int lane = threadIdx.x & 63;
+  const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+  const long long i = wave * TILE + lane - 1;
+  const long long n = a.n, pitch = a.pitch;
+
+  const double tcur = a.tring[a.step & 1];
+  if (!(tcur < a.t_end)) {  // main.jl:202 `while t < T`
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.tring[(a.step + 1) & 1] = tcur;
+    return;
+  }
+  const double lmax = __longlong_as_double(
+      (long long)__hip_atomic_load(a.slots + (a.step % 3), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  const double dt = a.cfl * a.dx / lmax;  // main.jl:212
+  const double dtdx = dt / a.dx;
+
+  const bool inr = (i >= 0) && (i < n);
+  const long long ic = i < 0 ? 0 : (i >= n ? n - 1 : i);
+  double q[NC], cc[NPH];
+#pragma unroll
+  for (int r = 0; r < NC; ++r) q[r] = a.Qin[r * pitch + ic];
+#pragma unroll
+  for (int p = 0; p < NPH; ++p) cc[p] = a.Cin[p * pitch + ic];
+
+  double f[NC], smin, smax;
+  int bad;
+  cell_flux<NPH>(a.mat, q, cc, f, smin, smax, bad);
+
+  // right neighbour (cell i+1) from lane+1
+  double qn[NC], fn[NC];
+#pragma unroll
+  for (int r = 0; r < NC; ++r) {
+    qn[r] = __shfl_down(q[r], 1);
+    fn[r] = __shfl_down(f[r], 1);
+  }
+  double smin_n = __shfl_down(smin, 1), smax_n = __shfl_down(smax, 1);
+
+  // face i+1/2 (this lane) and face i-1/2 (from lane-1)
+  double upd[NC];
+  if (FLUX == FLUX_RUSANOV) {
+    double lam_c = fmax(fabs(smin), fabs(smax));       // maximum(abs.(eigvals))
+    double lam_n = fmax(fabs(smin_n), fabs(smax_n));
+    double cons[NC], dmp[NC];
+    face_rusanov<NPH>(a.mat, q, qn, f, fn, fmax(lam_c, lam_n), cons, dmp);
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+      double cl = __shfl_up(cons[r], 1);
+      double nfr = NPH == 2 ? dmp[r] : 0.0;
+      double nfl = NPH == 2 ? __shfl_up(dmp[r], 1) : 0.0;
+      upd[r] = q[r] - dtdx * ((cons[r] - cl) + (nfr + nfl));  // main.jl:59
+    }
+  } else {
+    double dm[NC], dp[NC];
+    face_hll<NPH>(a.mat, q, qn, f, fn, smin, smax_n, dm, dp);
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+      double dpl = __shfl_up(dp[r], 1);
+      upd[r] = q[r] - dtdx * (dm[r] + dpl);  // conservative part is zero (NumFluxes.jl:82)
+    }
+  }
+
+  const bool owned = inr && lane >= 1 && lane <= TILE;
+  const bool interior = owned && i >= 1 && i <= n - 2;
+  double lam_new = 0.0;
+  int bad2 = 0;
+  if (interior) {
+#pragma unroll
+    for (int r = 0; r < NC; ++r) a.Qout[r * pitch + i] = upd[r];
+    double cn[NPH];
+    lam_new = cell_speeds<NPH>(a.mat, upd, cn, bad2);
+#pragma unroll
+    for (int p = 0; p < NPH; ++p) a.Cout[p * pitch + i] = cn[p];
+  } else if (owned) {  // frozen end cells (main.jl:219-220)
+    lam_new = fmax(fabs(smin), fabs(smax));
+#pragma unroll
+    for (int p = 0; p < NPH; ++p) a.Cout[p * pitch + i] = cc[p];
+  }
+  int anybad = (owned && bad) || bad2;
+  if (__any(anybad) && lane == 0) atomicOr(a.status, 1);
+
+  block_max_to_slot(lam_new, a.slots + ((a.step + 1) % 3));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.tring[(a.step + 1) & 1] = tcur + dt;
+    a.slots[(a.step + 2) % 3] = 0ull;
+    if (a.dt_hist) a.dt_hist[a.hist_idx] = dt;
+    atomicAdd(a.status + 1, 1);
+  }
+}
+
+// Sound speeds of every cell of a state and the max CFL speed (main.jl:204-212).
+template <int NPH>
+__global__ __launch_bounds__(BLOCK) void k_speeds(const double* __restrict__ Q,
+                                                  double* __restrict__ C, long long n,
+                                                  long long pitch, MatDev m0, MatDev m1,
+                                                  unsigned long long* slot, int* status) {
+  constexpr int NC = NComp<NPH>::value;
+  const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  MatDev mat[2] = {m0, m1};
+  double lam = 0.0;
+  int bad = 0;
+  if (i < n) {
+    double q[NC], cc[NPH];
+#pragma unroll
+    for (int r = 0; r < NC; ++r) q[r] = Q[r * pitch + i];
+    lam = cell_speeds<NPH>(mat, q, cc, bad);
+#pragma unroll
+    for (int p = 0; p < NPH; ++p) C[p * pitch + i] = cc[p];
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1);
+  block_max_to_slot(lam, slot);
+}
+
+// AoS (component fastest, the Julia Matrix layout) <-> SoA.
+__global__ void k_aos_to_soa(const double* __restrict__ in, double* __restrict__ out, int nc,
+                             long long n, long long pitch) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < nc; ++r) out[r * pitch + i] = in[i * nc + r];
+}
+__global__ void k_soa_to_aos(const double* __restrict__ in, double* __restrict__ out, int nc,
+                             long long n, long long pitch) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < nc; ++r) out[i * nc + r] = in[r * pitch + i];
+}
+
+// Piecewise-constant initial state: cell i (0-based, centre x = (i + 1/2) / n)
+// takes state k = #{j : x > s_j} for sorted interfaces s_j.  With one interface
+// at 1/2 this is main.jl:103 (cell i is left iff i < n/2).
+__global__ void k_piecewise(const double* __restrict__ states, const double* __restrict__ ifc,
+                            int nifc, int nc, double* __restrict__ Q, long long n,
+                            long long pitch) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = ((double)i + 0.5) / (double)n;
+  int lo = 0, hi = nifc;  // first j with !(x > s_j)
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (x > ifc[mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  const double* src = states + (long long)lo * nc;
+  for (int r = 0; r < nc; ++r) Q[r * pitch + i] = src[r];
+}
+
+// ---------------------------------------------------------------------------
+// Point-wise kernels behind the module API (one thread per point, AoS data)
+// ---------------------------------------------------------------------------
+enum PointOp {
+  OP_CONS2PRIM = 0,
+  OP_PRIM2CONS,
+  OP_FLUX,
+  OP_EIGVALS,
+  OP_NONCONS,
+  OP_MAXSPEED,
+};
+
+template <int NPH>
+__global__ __launch_bounds__(64) void k_point(int op, const double* __restrict__ in,
+                                              double* __restrict__ out, long long n, MatDev m0,
+                                              MatDev m1, double n0, double n1, double n2,
+                                              int* status) {
+  constexpr int NC = NComp<NPH>::value;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  MatDev mat[2] = {m0, m1};
+  double q[NC];
+#pragma unroll
+  for (int r = 0; r < NC; ++r) q[r] = in[i * NC + r];
+  int bad = 0;
+  if (op == OP_CONS2PRIM) {  // HyperelasticityMPh.jl:99-133
+    double* o = out + i * NC;
+    for (int p = 0; p < NPH; ++p) {
+      Thermo t;
+      phase_thermo<NPH>(mat, q, p, t);
+      const MatDev& m = mat[NPH == 1 ? 0 : p];
+      int b0 = NPH == 1 ? -1 : 15 * p;
+      if (NPH == 2) o[b0] = q[b0];
+      o[b0 + 1] = t.rho;
+      o[b0 + 2] = t.u[0];
+      o[b0 + 3] = t.u[1];
+      o[b0 + 4] = t.u[2];
+      o[b0 + 5] = entropy_of(m, t);
+      for (int k = 0; k < 9; ++k) o[b0 + 6 + k] = t.A[k];
+      bad |= t.bad;
+    }
+  } else if (op == OP_PRIM2CONS) {  // HyperelasticityMPh.jl:63-90
+    double* o = out + i * NC;
+    for (int p = 0; p < NPH; ++p) {
+      const MatDev& m = mat[NPH == 1 ? 0 : p];
+      int b0 = NPH == 1 ? -1 : 15 * p;
+      double frac = NPH == 1 ? 1.0 : q[b0];
+      double true_den = q[b0 + 1];
+      double den = frac * true_den;
+      const double* vel = q + b0 + 2;
+      double S = q[b0 + 5];
+      const double* A = q + b0 + 6;
+      double e_int = energy_from_A(m, A, exp(S / m.cv));
+      double e_kin = (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]) / 2.0;
+      if (NPH == 2) o[b0] = frac;
+      o[b0 + 1] = den;
+      for (int k = 0; k < 3; ++k) o[b0 + 2 + k] = den * vel[k];
+      o[b0 + 5] = den * (e_int + e_kin);
+      for (int k = 0; k < 9; ++k) o[b0 + 6 + k] = den * A[k];
+    }
+  } else if (op == OP_FLUX) {  // HyperelasticityMPh.jl:140-175
+    double cc[2] = {0.0, 0.0}, f[NC], smin, smax;
+    cell_flux<NPH>(mat, q, cc, f, smin, smax, bad);
+    for (int r = 0; r < NC; ++r) out[i * NC + r] = f[r];
+  } else if (op == OP_EIGVALS) {  // HyperelasticityMPh.jl:252-266 -> 6 per phase
+    for (int p = 0; p < NPH; ++p) {
+      Thermo t;
+      phase_thermo<NPH>(mat, q, p, t);
+      const double nv[3] = {n0, n1, n2};
+      double O9[9], w[3];
+      acoustic_n(mat[NPH == 1 ? 0 : p], t, nv, O9);
+      double O[6] = {O9[0], 0.5 * (O9[1] + O9[3]), 0.5 * (O9[2] + O9[6]), O9[4],
+                     0.5 * (O9[5] + O9[7]), O9[8]};
+      eig3_sym(O, w);
+      double spd = t.u[0] * n0 + t.u[1] * n1 + t.u[2] * n2;  // dot(P[3:5], n)
+      double* o = out + i * 6 * NPH + 6 * p;
+      for (int k = 0; k < 3; ++k) {
+        double c = sqrt(fabs(w[k]));
+        o[k] = spd + c;
+        o[3 + k] = spd - c;
+      }
+      bad |= t.bad;
+    }
+  } else if (op == OP_NONCONS && NPH == 2) {  // HyperelasticityMPh.jl:178-250 -> 30x30
+    BCol b;
+    bcol2(mat, q, b);
+    double* o = out + i * 900;
+    for (int k = 0; k < 900; ++k) o[k] = 0.0;
+    for (int p = 0; p < 2; ++p) {
+      int off = 15 * p;
+      double col[15];
+      col[0] = b.uI0;
+      col[1] = 0.0;
+      col[2] = b.sI[0];
+      col[3] = b.sI[1];
+      col[4] = b.sI[2];
+      col[5] = b.sIu;
+      for (int r = 0; r < 9; ++r) col[6 + r] = b.a[p][r];
+      for (int r = 0; r < 15; ++r) o[(off + r) + 30 * off] = col[r];
+    }
+  } else if (op == OP_MAXSPEED) {
+    double cc[NPH];
+    out[i] = cell_speeds<NPH>(mat, q, cc, bad);
+  }
+  if (bad) atomicOr(status, 1);
+}
+
+// EquationsOfState point functions: energy / entropy / stress / acoustic.
+enum EosOp { EOS_ENERGY = 0, EOS_ENTROPY, EOS_STRESS, EOS_STRESS_DEN, EOS_ACOUSTIC, EOS_ACOUSTIC_PRIM,
+             EOS_TEMPERATURE, EOS_FINGER, EOS_INVARIANTS };
+
+__global__ __launch_bounds__(64) void k_eos(int op, MatDev m, const double* __restrict__ s, const double* __restrict__ s2,
+                      const double* __restrict__ v9, const double* __restrict__ nv,
+                      double* __restrict__ out, long long n, int* status) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int bad = 0;
+  if (op == EOS_ENERGY) {  // energy(eos, S, G), EquationsOfState.jl:118
+    out[i] = energy_G(m, s[i], v9 + 9 * i);
+  } else if (op == EOS_ENTROPY) {  // entropy(eos, e_int, G), :139
+    out[i] = entropy_G(m, s[i], v9 + 9 * i);
+  } else if (op == EOS_TEMPERATURE) {  // d energy / dS
+    const double* g = v9 + 9 * i;
+    double inv[3];
+    invariants_gen(g, inv);
+    out[i] = m.t0 * exp(m.half_g * log(inv[2])) * exp(s[i] / m.cv);
+  } else if (op == EOS_STRESS || op == EOS_ACOUSTIC) {  // stress(eos, ent, F) :179 / acoustic :223
+    Thermo t;
+    thermo_from_A(m, v9 + 9 * i, exp(s[i] / m.cv), t);
+    bad = t.bad;
+    if (op == EOS_STRESS) {
+      stress_full(m, t, out + 9 * i);
+    } else {
+      acoustic_n(m, t, nv + 3 * i, out + 9 * i);
+    }
+  } else if (op == EOS_STRESS_DEN) {  // stress(eos, den, e_int, F) :162
+    const double* F = v9 + 9 * i;
+    double G[9];
+    {
+      Thermo t;
+      thermo_from_A(m, F, 1.0, t);
+      G[0] = t.G11; G[1] = t.G12; G[2] = t.G13;
+      G[3] = t.G12; G[4] = t.G22; G[5] = t.G23;
+      G[6] = t.G13; G[7] = t.G23; G[8] = t.G33;
+    }
+    double S = entropy_G(m, s2[i], G);
+    Thermo t;
+    thermo_from_A(m, F, exp(S / m.cv), t);
+    t.rho = s[i];  // the density argument, as the reference uses it
+    bad = t.bad;
+    stress_full(m, t, out + 9 * i);
+  } else if (op == EOS_ACOUSTIC_PRIM) {  // acoustic(eos, P, n) :192 ; v9 holds P (15)
+    const double* P = v9 + 15 * i;
+    const double* F = P + 6;
+    // e_int = energy(eos, P[6], G); the stress then recomputes the entropy from
+    // e_int, so x is the entropy law's argument (clamped at 1e-6).
+    Thermo t;
+    double x0 = exp(P[5] / m.cv);
+    thermo_from_A(m, F, x0, t);
+    double sam1 = t.sa - 1.0;
+    double Uc = m.hk0a2 * sam1 * sam1;
+    double e_int = Uc + t.Eth + t.W;
+    double x = (e_int - t.W - Uc) / (m.cvt0 * t.sg) + 1.0;
+    bool clamped = x < 1e-6;
+    if (clamped) x = 1e-6;
+    t.x = x;
+    t.Eth = m.cvt0 * t.sg * (x - 1.0);
+    t.rho = P[0] * P[1];  // den = P[1] * P[2], held fixed
+    acoustic_n(m, t, nv + 3 * i, out + 9 * i, true, clamped);
+    bad = t.bad;
+  } else if (op == EOS_FINGER) {  // Strains.jl:26-32
+    Thermo t;
+    thermo_from_A(m, v9 + 9 * i, 1.0, t);
+    double* g = out + 9 * i;
+    g[0] = t.G11; g[1] = t.G12; g[2] = t.G13;
+    g[3] = t.G12; g[4] = t.G22; g[5] = t.G23;
+    g[6] = t.G13; g[7] = t.G23; g[8] = t.G33;
+  } else if (op == EOS_INVARIANTS) {  // Strains.jl:46-52
+    invariants_gen(v9 + 9 * i, out + 3 * i);
+  }
+  if (bad) atomicOr(status, 1);
+}
+
+// Face fluxes for arrays of face states (NumFluxes.lxf / hll).
+template <int NPH, int FLUX>
+__global__ __launch_bounds__(64) void k_face(const double* __restrict__ QL, const double* __restrict__ QR,
+                       const double* __restrict__ aux, double* __restrict__ cons,
+                       double* __restrict__ dm, double* __restrict__ dp, long long n, MatDev m0,
+                       MatDev m1) {
+  constexpr int NC = NComp<NPH>::value;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  MatDev mat[2] = {m0, m1};
+  double ql[NC], qr[NC], fl[NC], fr[NC];
+#pragma unroll
+  for (int r = 0; r < NC; ++r) {
+    ql[r] = QL[i * NC + r];
+    qr[r] = QR[i * NC + r];
+  }
+  double cc[2] = {0.0, 0.0}, s0, s1;
+  int bad;
+  cell_flux<NPH>(mat, ql, cc, fl, s0, s1, bad);
+  cell_flux<NPH>(mat, qr, cc, fr, s0, s1, bad);
+  if (FLUX == FLUX_RUSANOV) {
+    double c[NC], d[NC];
+    face_rusanov<NPH>(mat, ql, qr, fl, fr, aux[i], c, d);
+    for (int r = 0; r < NC; ++r) {
+      cons[i * NC + r] = c[r];
+      dm[i * NC + r] = NPH == 2 ? d[r] : 0.0;
+      dp[i * NC + r] = NPH == 2 ? d[r] : 0.0;
+    }
+  } else {
+    // aux holds [min eig of left, max eig of right] per face
+    double a[NC], b[NC];
+    face_hll<NPH>(mat, ql, qr, fl, fr, aux[2 * i], aux[2 * i + 1], a, b);
+    for (int r = 0; r < NC; ++r) {
+      cons[i * NC + r] = 0.0;
+      dm[i * NC + r] = a[r];
+      dp[i * NC + r] = b[r];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Solver handle
+// ---------------------------------------------------------------------------
+struct hes_solver {
+  int nph, flux, nc;
+  long long n, pitch;
+  double mats[16];
+  MatDev mat[2];
+  double cfl, dx;
+  double* Q[2];
+  double* C[2];
+  unsigned long long* slots;
+  double* tring;
+  int* status;
+  double* dt_hist;
+  long long dt_cap;
+  long long step;       // steps executed so far (state index)
+  double t;             // time of the current state
+  hipStream_t stream;
+  hipEvent_t ev0, ev1;
+  bool has_state;
+};
+
+namespace {
+
+template <int NPH, int FLUX>
+void launch_step_t(const StepArgs& a, long long nwaves, hipStream_t s) {
+  long long nblocks = (nwaves + 3) / 4;
+  hipLaunchKernelGGL((k_step<NPH, FLUX>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
+}
+
+void launch_step(hes_solver* h, const StepArgs& a) {
+  long long nwaves = (h->n + TILE - 1) / TILE;
+  if (h->nph == 1 && h->flux == FLUX_RUSANOV) launch_step_t<1, FLUX_RUSANOV>(a, nwaves, h->stream);
+  else if (h->nph == 1) launch_step_t<1, FLUX_HLL>(a, nwaves, h->stream);
+  else if (h->flux == FLUX_RUSANOV) launch_step_t<2, FLUX_RUSANOV>(a, nwaves, h->stream);
+  else launch_step_t<2, FLUX_HLL>(a, nwaves, h->stream);
+}
+
+StepArgs make_args(hes_solver* h, long long s, double t_end, double* hist, int hist_idx) {
+  StepArgs a;
+  a.Qin = h->Q[s & 1];
+  a.Qout = h->Q[(s + 1) & 1];
+  a.Cin = h->C[s & 1];
+  a.Cout = h->C[(s + 1) & 1];
+  a.slots = h->slots;
+  a.tring = h->tring;
+  a.dt_hist = hist;
+  a.status = h->status;
+  a.n = h->n;
+  a.pitch = h->pitch;
+  a.cfl = h->cfl;
+  a.dx = h->dx;
+  a.t_end = t_end;
+  a.step = (int)(s % 6);
+  a.hist_idx = hist_idx;
+  a.mat[0] = h->mat[0];
+  a.mat[1] = h->mat[1];
+  return a;
+}
+
+// Recompute sound speeds and the CFL max of the current state; reset rings.
+int init_speeds(hes_solver* h) {
+  long long s = h->step;
+  HIP_TRY(hipMemsetAsync(h->slots, 0, 3 * sizeof(unsigned long long), h->stream));
+  double tr[2] = {h->t, h->t};
+  HIP_TRY(hipMemcpyAsync(h->tring, tr, sizeof(tr), hipMemcpyHostToDevice, h->stream));
+  long long nb = (h->n + BLOCK - 1) / BLOCK;
+  const double* Qs = h->Q[s & 1];
+  double* Cs = h->C[s & 1];
+  if (h->nph == 1)
+    hipLaunchKernelGGL(k_speeds<1>, dim3((unsigned)nb), dim3(BLOCK), 0, h->stream, Qs, Cs, h->n,
+                       h->pitch, h->mat[0], h->mat[1], h->slots + (s % 3), h->status);
+  else
+    hipLaunchKernelGGL(k_speeds<2>, dim3((unsigned)nb), dim3(BLOCK), 0, h->stream, Qs, Cs, h->n,
+                       h->pitch, h->mat[0], h->mat[1], h->slots + (s % 3), h->status);
+  HIP_TRY(hipGetLastError());
+  // the end cells never change: copy their state and speeds into the other buffer
+  for (int r = 0; r < h->nc; ++r) {
+    HIP_TRY(hipMemcpyAsync(h->Q[(s + 1) & 1] + r * h->pitch, h->Q[s & 1] + r * h->pitch,
+                           sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->Q[(s + 1) & 1] + r * h->pitch + h->n - 1,
+                           h->Q[s & 1] + r * h->pitch + h->n - 1, sizeof(double),
+                           hipMemcpyDeviceToDevice, h->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  int st[2];
+  HIP_TRY(hipMemcpy(st, h->status, sizeof(st), hipMemcpyDeviceToHost));
+  if (st[0]) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
+  return HES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hes_last_error(void) { return g_err.c_str(); }
+
+int hes_version(void) { return HES_ABI_VERSION; }
+
+int hes_device_info(char* name, int len, int* cus, double* mem_gb) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  HIP_TRY(hipGetDeviceProperties(&p, dev));
+  if (name && len > 0) {
+    std::snprintf(name, (size_t)len, "%s", p.gcnArchName);
+  }
+  if (cus) *cus = p.multiProcessorCount;
+  if (mem_gb) *mem_gb = (double)p.totalGlobalMem / 1e9;
+  return HES_OK;
+}
+
+int hes_solver_create(hes_solver** out, int nphase, int flux, long long ncells,
+                      const double* materials, double cfl, double dx) {
+  if (!out) return fail(HES_ERR_ARG, "null handle pointer");
+  *out = nullptr;
+  if (nphase != 1 && nphase != 2) return fail(HES_ERR_ARG, "nphase must be 1 or 2");
+  if (flux != HES_FLUX_RUSANOV && flux != HES_FLUX_HLL) return fail(HES_ERR_ARG, "unknown flux");
+  if (ncells < 3) return fail(HES_ERR_ARG, "need at least 3 cells");
+  if (!materials) return fail(HES_ERR_ARG, "null materials");
+  for (int p = 0; p < nphase; ++p)
+    if (!mat_ok(materials + 8 * p)) return fail(HES_ERR_ARG, "invalid material constants");
+  if (!(cfl > 0) || !(dx > 0)) return fail(HES_ERR_ARG, "cfl and dx must be positive");
+  hes_solver* h = new hes_solver();
+  std::memset(h, 0, sizeof(*h));
+  h->nph = nphase;
+  h->flux = flux;
+  h->nc = nphase == 1 ? 14 : 30;
+  h->n = ncells;
+  h->pitch = (ncells + 63) / 64 * 64;
+  for (int k = 0; k < 8 * nphase; ++k) h->mats[k] = materials[k];
+  h->mat[0] = make_mat(materials);
+  h->mat[1] = make_mat(nphase == 2 ? materials + 8 : materials);
+  h->cfl = cfl;
+  h->dx = dx;
+  size_t qbytes = sizeof(double) * (size_t)h->nc * (size_t)h->pitch;
+  size_t cbytes = sizeof(double) * (size_t)nphase * (size_t)h->pitch;
+  hipError_t e = hipSuccess;
+  for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+    e = hipMalloc(&h->Q[b], qbytes);
+    if (e == hipSuccess) e = hipMalloc(&h->C[b], cbytes);
+  }
+  if (e == hipSuccess) e = hipMalloc(&h->slots, 4 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc(&h->tring, 2 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&h->status, 4 * sizeof(int));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e == hipSuccess) e = hipMemset(h->status, 0, 4 * sizeof(int));
+  if (e != hipSuccess) {
+    g_err = std::string("allocation failed: ") + hipGetErrorString(e);
+    hes_solver_destroy(h);
+    return HES_ERR_HIP;
+  }
+  *out = h;
+  return HES_OK;
+}
+
+int hes_solver_destroy(hes_solver* h) {
+  if (!h) return HES_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (int b = 0; b < 2; ++b) {
+    if (h->Q[b]) (void)hipFree(h->Q[b]);
+    if (h->C[b]) (void)hipFree(h->C[b]);
+  }
+  if (h->slots) (void)hipFree(h->slots);
+  if (h->tring) (void)hipFree(h->tring);
+  if (h->status) (void)hipFree(h->status);
+  if (h->dt_hist) (void)hipFree(h->dt_hist);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return HES_OK;
+}
+
+static int set_state_common(hes_solver* h, const double* src, hipMemcpyKind kind, double t0) {
+  size_t bytes = sizeof(double) * (size_t)h->nc * (size_t)h->n;
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, bytes));
+  hipError_t e = hipMemcpyAsync(tmp, src, bytes, kind, h->stream);
+  if (e == hipSuccess) {
+    h->step = 0;
+    h->t = t0;
+    long long nb = (h->n + 255) / 256;
+    hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)nb), dim3(256), 0, h->stream, tmp, h->Q[0],
+                       h->nc, h->n, h->pitch);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) {
+    g_err = std::string("set_state: ") + hipGetErrorString(e);
+    return HES_ERR_HIP;
+  }
+  HIP_TRY(hipMemset(h->status, 0, 4 * sizeof(int)));
+  h->has_state = true;
+  return init_speeds(h);
+}
+
+int hes_solver_set_state(hes_solver* h, const double* Q, double t0) {
+  if (!h || !Q) return fail(HES_ERR_ARG, "null argument");
+  return set_state_common(h, Q, hipMemcpyHostToDevice, t0);
+}
+
+int hes_solver_set_piecewise(hes_solver* h, int nstates, const double* states,
+                             const double* interfaces, double t0) {
+  if (!h || !states || nstates < 1 || (nstates > 1 && !interfaces))
+    return fail(HES_ERR_ARG, "null argument");
+  for (int j = 0; j + 1 < nstates - 1; ++j)
+    if (!(interfaces[j] <= interfaces[j + 1])) return fail(HES_ERR_ARG, "interfaces not sorted");
+  double *ds = nullptr, *di = nullptr;
+  HIP_TRY(hipMalloc(&ds, sizeof(double) * (size_t)nstates * h->nc));
+  HIP_TRY(hipMalloc(&di, sizeof(double) * (size_t)(nstates > 1 ? nstates - 1 : 1)));
+  hipError_t e = hipMemcpy(ds, states, sizeof(double) * (size_t)nstates * h->nc, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nstates > 1)
+    e = hipMemcpy(di, interfaces, sizeof(double) * (size_t)(nstates - 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    h->step = 0;
+    h->t = t0;
+    long long nb = (h->n + 255) / 256;
+    hipLaunchKernelGGL(k_piecewise, dim3((unsigned)nb), dim3(256), 0, h->stream, ds, di,
+                       nstates - 1, h->nc, h->Q[0], h->n, h->pitch);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(ds);
+  (void)hipFree(di);
+  if (e != hipSuccess) {
+    g_err = std::string("set_piecewise: ") + hipGetErrorString(e);
+    return HES_ERR_HIP;
+  }
+  HIP_TRY(hipMemset(h->status, 0, 4 * sizeof(int)));
+  h->has_state = true;
+  return init_speeds(h);
+}
+
+int hes_solver_set_state_device(hes_solver* h, const double* dQ, double t0) {
+  if (!h || !dQ) return fail(HES_ERR_ARG, "null argument");
+  return set_state_common(h, dQ, hipMemcpyDeviceToDevice, t0);
+}
+
+int hes_solver_get_state(hes_solver* h, double* Q) {
+  if (!h || !Q) return fail(HES_ERR_ARG, "null argument");
+  if (!h->has_state) return fail(HES_ERR_ARG, "no state set");
+  size_t bytes = sizeof(double) * (size_t)h->nc * (size_t)h->n;
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, bytes));
+  long long nb = (h->n + 255) / 256;
+  hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)nb), dim3(256), 0, h->stream, h->Q[h->step & 1],
+                     tmp, h->nc, h->n, h->pitch);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(Q, tmp, bytes, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) {
+    g_err = std::string("get_state: ") + hipGetErrorString(e);
+    return HES_ERR_HIP;
+  }
+  return HES_OK;
+}
+
+int hes_solver_get_speeds(hes_solver* h, double* c) {
+  if (!h || !c) return fail(HES_ERR_ARG, "null argument");
+  for (int p = 0; p < h->nph; ++p)
+    HIP_TRY(hipMemcpy(c + p * h->n, h->C[h->step & 1] + p * h->pitch, sizeof(double) * h->n,
+                      hipMemcpyDeviceToHost));
+  return HES_OK;
+}
+
+int hes_solver_time(hes_solver* h, double* t, long long* steps) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  if (t) *t = h->t;
+  if (steps) *steps = h->step;
+  return HES_OK;
+}
+
+int hes_solver_next_dt(hes_solver* h, double* dt) {
+  if (!h || !dt) return fail(HES_ERR_ARG, "null argument");
+  unsigned long long bits = 0;
+  HIP_TRY(hipMemcpy(&bits, h->slots + (h->step % 3), sizeof(bits), hipMemcpyDeviceToHost));
+  double lmax;
+  std::memcpy(&lmax, &bits, sizeof(lmax));
+  *dt = h->cfl * h->dx / lmax;
+  return HES_OK;
+}
+
+// Advance up to max_steps steps while t < t_end.  Launches are issued in
+// chunks of `chunk` steps (captured once into a hipGraph when use_graph is
+// set) and the executed-step counter is read back after each chunk.
+int hes_solver_run(hes_solver* h, long long max_steps, double t_end, int use_graph,
+                   long long* steps_done, double* t_now, float* elapsed_ms, double* dt_out) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  if (!h->has_state) return fail(HES_ERR_ARG, "no state set");
+  if (max_steps < 0) return fail(HES_ERR_ARG, "negative step count");
+  const long long chunk = 60;  // multiple of 6: ring phases repeat per chunk
+  if (dt_out) {
+    if (h->dt_cap < max_steps) {
+      if (h->dt_hist) HIP_TRY(hipFree(h->dt_hist));
+      h->dt_hist = nullptr;
+      h->dt_cap = 0;
+      HIP_TRY(hipMalloc(&h->dt_hist, sizeof(double) * (size_t)(max_steps > 0 ? max_steps : 1)));
+      h->dt_cap = max_steps;
+    }
+  }
+  HIP_TRY(hipMemsetAsync(h->status + 1, 0, sizeof(int), h->stream));
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  long long graph_base = -1;  // step (mod 6) the captured graph starts from
+  long long launched = 0, done = 0;
+  const long long s0 = h->step;
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  int rc = HES_OK;
+  while (launched < max_steps) {
+    long long k = max_steps - launched < chunk ? max_steps - launched : chunk;
+    long long sb = s0 + launched;
+    bool graph_ok = use_graph && k == chunk && !dt_out;
+    if (graph_ok) {
+      if (!gexec || (graph_base % 6) != (sb % 6)) {
+        if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+        hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) { rc = fail(HES_ERR_HIP, "stream capture failed"); break; }
+        for (long long j = 0; j < k; ++j) launch_step(h, make_args(h, sb + j, t_end, nullptr, 0));
+        e = hipStreamEndCapture(h->stream, &graph);
+        if (e == hipSuccess) e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) { g_err = std::string("graph: ") + hipGetErrorString(e); rc = HES_ERR_HIP; break; }
+        graph_base = sb;
+      }
+      hipError_t e = hipGraphLaunch(gexec, h->stream);
+      if (e != hipSuccess) { g_err = std::string("graph launch: ") + hipGetErrorString(e); rc = HES_ERR_HIP; break; }
+    } else {
+      for (long long j = 0; j < k; ++j)
+        launch_step(h, make_args(h, sb + j, t_end, dt_out ? h->dt_hist : nullptr, (int)(launched + j)));
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) { g_err = std::string("launch: ") + hipGetErrorString(e); rc = HES_ERR_HIP; break; }
+    }
+    launched += k;
+    if (t_end < INFINITY || launched >= max_steps) {
+      int cnt = 0;
+      hipError_t e = hipMemcpyAsync(&cnt, h->status + 1, sizeof(int), hipMemcpyDeviceToHost, h->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) { g_err = std::string("sync: ") + hipGetErrorString(e); rc = HES_ERR_HIP; break; }
+      done = cnt;
+      if (done < launched) break;  // t reached t_end inside this chunk
+    }
+  }
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc != HES_OK) return rc;
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  int st[2];
+  HIP_TRY(hipMemcpy(st, h->status, sizeof(st), hipMemcpyDeviceToHost));
+  done = st[1];
+  h->step = s0 + done;
+  double tr[2];
+  HIP_TRY(hipMemcpy(tr, h->tring, sizeof(tr), hipMemcpyDeviceToHost));
+  h->t = tr[h->step & 1];
+  if (steps_done) *steps_done = done;
+  if (t_now) *t_now = h->t;
+  if (elapsed_ms) *elapsed_ms = ms;
+  if (dt_out && done > 0)
+    HIP_TRY(hipMemcpy(dt_out, h->dt_hist, sizeof(double) * (size_t)done, hipMemcpyDeviceToHost));
+  if (st[0]) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
+  return HES_OK;
+}
+
+// Time `reps` launches of the step kernel alone on the solver's stream (HIP
+// events bracket exactly these launches).  The state advances by reps steps.
+int hes_solver_time_steps(hes_solver* h, long long reps, float* elapsed_ms) {
+  if (!h || !elapsed_ms) return fail(HES_ERR_ARG, "null argument");
+  if (!h->has_state) return fail(HES_ERR_ARG, "no state set");
+  HIP_TRY(hipMemsetAsync(h->status + 1, 0, sizeof(int), h->stream));
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  for (long long j = 0; j < reps; ++j) launch_step(h, make_args(h, h->step + j, INFINITY, nullptr, 0));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  HIP_TRY(hipEventElapsedTime(elapsed_ms, h->ev0, h->ev1));
+  int st[2];
+  HIP_TRY(hipMemcpy(st, h->status, sizeof(st), hipMemcpyDeviceToHost));
+  h->step += st[1];
+  double tr[2];
+  HIP_TRY(hipMemcpy(tr, h->tring, sizeof(tr), hipMemcpyDeviceToHost));
+  h->t = tr[h->step & 1];
+  if (st[0]) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
+  return HES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Point-wise API (host arrays, synchronous)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DevBuf {
+  double* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t alloc(size_t n) { return hipMalloc(&p, n * sizeof(double) + 8); }
+};
+
+struct DevStatus {
+  int* p = nullptr;
+  ~DevStatus() { if (p) (void)hipFree(p); }
+};
+
+int finish_status(DevStatus& st) {
+  int v = 0;
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(&v, st.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (v) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
+  return HES_OK;
+}
+
+int point_call(int op, int nph, const double* mats, const double* in, double* out, long long n,
+               int in_w, int out_w, const double* nvec = nullptr) {
+  if (nph != 1 && nph != 2) return fail(HES_ERR_ARG, "nphase must be 1 or 2");
+  if (!mats || !in || !out || n < 0) return fail(HES_ERR_ARG, "null argument");
+  for (int p = 0; p < nph; ++p)
+    if (!mat_ok(mats + 8 * p)) return fail(HES_ERR_ARG, "invalid material constants");
+  if (n == 0) return HES_OK;
+  MatDev m0 = make_mat(mats), m1 = make_mat(nph == 2 ? mats + 8 : mats);
+  DevBuf di, dout;
+  DevStatus st;
+  HIP_TRY(di.alloc((size_t)n * in_w));
+  HIP_TRY(dout.alloc((size_t)n * out_w));
+  HIP_TRY(hipMalloc(&st.p, sizeof(int)));
+  HIP_TRY(hipMemset(st.p, 0, sizeof(int)));
+  HIP_TRY(hipMemcpy(di.p, in, sizeof(double) * (size_t)n * in_w, hipMemcpyHostToDevice));
+  unsigned nb = (unsigned)((n + 63) / 64);
+  double nv[3] = {1.0, 0.0, 0.0};
+  if (nvec) { nv[0] = nvec[0]; nv[1] = nvec[1]; nv[2] = nvec[2]; }
+  if (nph == 1)
+    hipLaunchKernelGGL(k_point<1>, dim3(nb), dim3(64), 0, 0, op, di.p, dout.p, n, m0, m1, nv[0],
+                       nv[1], nv[2], st.p);
+  else
+    hipLaunchKernelGGL(k_point<2>, dim3(nb), dim3(64), 0, 0, op, di.p, dout.p, n, m0, m1, nv[0],
+                       nv[1], nv[2], st.p);
+  HIP_TRY(hipGetLastError());
+  int rc = finish_status(st);
+  HIP_TRY(hipMemcpy(out, dout.p, sizeof(double) * (size_t)n * out_w, hipMemcpyDeviceToHost));
+  return rc;
+}
+
+int eos_call(int op, const double* mat, const double* s, const double* s2, const double* v,
+             int vw, const double* nv, double* out, int ow, long long n) {
+  if (!mat || !out || n < 0) return fail(HES_ERR_ARG, "null argument");
+  if (!mat_ok(mat)) return fail(HES_ERR_ARG, "invalid material constants");
+  if (n == 0) return HES_OK;
+  MatDev m = make_mat(mat);
+  DevBuf ds, ds2, dv, dn, dout;
+  DevStatus st;
+  HIP_TRY(hipMalloc(&st.p, sizeof(int)));
+  HIP_TRY(hipMemset(st.p, 0, sizeof(int)));
+  if (s) { HIP_TRY(ds.alloc(n)); HIP_TRY(hipMemcpy(ds.p, s, sizeof(double) * n, hipMemcpyHostToDevice)); }
+  if (s2) { HIP_TRY(ds2.alloc(n)); HIP_TRY(hipMemcpy(ds2.p, s2, sizeof(double) * n, hipMemcpyHostToDevice)); }
+  if (v) { HIP_TRY(dv.alloc((size_t)n * vw)); HIP_TRY(hipMemcpy(dv.p, v, sizeof(double) * n * vw, hipMemcpyHostToDevice)); }
+  if (nv) { HIP_TRY(dn.alloc((size_t)n * 3)); HIP_TRY(hipMemcpy(dn.p, nv, sizeof(double) * n * 3, hipMemcpyHostToDevice)); }
+  HIP_TRY(dout.alloc((size_t)n * ow));
+  unsigned nb = (unsigned)((n + 63) / 64);
+  hipLaunchKernelGGL(k_eos, dim3(nb), dim3(64), 0, 0, op, m, ds.p, ds2.p, dv.p, dn.p, dout.p, n, st.p);
+  HIP_TRY(hipGetLastError());
+  int rc = finish_status(st);
+  HIP_TRY(hipMemcpy(out, dout.p, sizeof(double) * (size_t)n * ow, hipMemcpyDeviceToHost));
+  return rc;
+}
+
+}  // namespace
+
+int hes_cons2prim(int nph, const double* mats, const double* Q, double* P, long long n) {
+  int w = nph == 1 ? 14 : 30;
+  return point_call(OP_CONS2PRIM, nph, mats, Q, P, n, w, w);
+}
+int hes_prim2cons(int nph, const double* mats, const double* P, double* Q, long long n) {
+  int w = nph == 1 ? 14 : 30;
+  return point_call(OP_PRIM2CONS, nph, mats, P, Q, n, w, w);
+}
+int hes_flux(int nph, const double* mats, const double* Q, double* F, long long n) {
+  int w = nph == 1 ? 14 : 30;
+  return point_call(OP_FLUX, nph, mats, Q, F, n, w, w);
+}
+int hes_get_eigvals(int nph, const double* mats, const double* Q, const double* nvec, double* eig,
+                    long long n) {
+  int w = nph == 1 ? 14 : 30;
+  return point_call(OP_EIGVALS, nph, mats, Q, eig, n, w, 6 * nph, nvec);
+}
+int hes_max_speed(int nph, const double* mats, const double* Q, double* lam, long long n) {
+  int w = nph == 1 ? 14 : 30;
+  return point_call(OP_MAXSPEED, nph, mats, Q, lam, n, w, 1);
+}
+int hes_noncons_flux(const double* mats, const double* Q, double* B, long long n) {
+  return point_call(OP_NONCONS, 2, mats, Q, B, n, 30, 900);
+}
+
+int hes_energy(const double* mat, const double* S, const double* G, double* e, long long n) {
+  if (!S || !G) return fail(HES_ERR_ARG, "null argument");
+  return eos_call(EOS_ENERGY, mat, S, nullptr, G, 9, nullptr, e, 1, n);
+}
+int hes_entropy(const double* mat, const double* e_int, const double* G, double* S, long long n) {
+  if (!e_int || !G) return fail(HES_ERR_ARG, "null argument");
+  return eos_call(EOS_ENTROPY, mat, e_int, nullptr, G, 9, nullptr, S, 1, n);
+}
+int hes_temperature(const double* mat, const double* S, const double* G, double* T, long long n) {
+  if (!S || !G) return fail(HES_ERR_ARG, "null argument");
+  return eos_call(EOS_TEMPERATURE, mat, S, nullptr, G, 9, nullptr, T, 1, n);
+}
+int hes_stress(const double* mat, const double* ent, const double* F, double* sig, long long n) {
+  if (!ent || !F) return fail(HES_ERR_ARG, "null argument");
+  return eos_call(EOS_STRESS, mat, ent, nullptr, F, 9, nullptr, sig, 9, n);
+}
+int hes_stress_den(const double* mat, const double* den, const double* e_int, const double* F,
+                   double* sig, long long n) {
+  if (!den || !e_int || !F) return fail(HES_ERR_ARG, "null argument");
+  return eos_call(EOS_STRESS_DEN, mat, den, e_int, F, 9, nullptr, sig, 9, n);
+}
+int hes_acoustic(const double* mat, const double* ent, const double* F, const double* nvec,
+                 double* ac, long long n) {
+  if (!ent || !F || !nvec) return fail(HES_ERR_ARG, "null argument");
+  return eos_call(EOS_ACOUSTIC, mat, ent, nullptr, F, 9, nvec, ac, 9, n);
+}
+int hes_acoustic_prim(const double* mat, const double* P, const double* nvec, double* ac,
+                      long long n) {
+  if (!P || !nvec) return fail(HES_ERR_ARG, "null argument");
+  return eos_call(EOS_ACOUSTIC_PRIM, mat, nullptr, nullptr, P, 15, nvec, ac, 9, n);
+}
+int hes_finger(const double* A, double* G, long long n) {
+  if (!A) return fail(HES_ERR_ARG, "null argument");
+  static const double dummy[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+  return eos_call(EOS_FINGER, dummy, nullptr, nullptr, A, 9, nullptr, G, 9, n);
+}
+int hes_invariants(const double* G, double* inv, long long n) {
+  if (!G) return fail(HES_ERR_ARG, "null argument");
+  static const double dummy[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+  return eos_call(EOS_INVARIANTS, dummy, nullptr, nullptr, G, 9, nullptr, inv, 3, n);
+}
+
+int hes_face_flux(int nph, int flux, const double* mats, const double* QL, const double* QR,
+                  const double* aux, double* cons, double* dm, double* dp, long long n) {
+  if (nph != 1 && nph != 2) return fail(HES_ERR_ARG, "nphase must be 1 or 2");
+  if (flux != HES_FLUX_RUSANOV && flux != HES_FLUX_HLL) return fail(HES_ERR_ARG, "unknown flux");
+  if (!mats || !QL || !QR || !aux || !cons || !dm || !dp || n < 0)
+    return fail(HES_ERR_ARG, "null argument");
+  for (int p = 0; p < nph; ++p)
+    if (!mat_ok(mats + 8 * p)) return fail(HES_ERR_ARG, "invalid material constants");
+  if (n == 0) return HES_OK;
+  int w = nph == 1 ? 14 : 30;
+  int aw = flux == HES_FLUX_HLL ? 2 : 1;
+  MatDev m0 = make_mat(mats), m1 = make_mat(nph == 2 ? mats + 8 : mats);
+  DevBuf dl, dr, da, dc, ddm, ddp;
+  HIP_TRY(dl.alloc((size_t)n * w));
+  HIP_TRY(dr.alloc((size_t)n * w));
+  HIP_TRY(da.alloc((size_t)n * aw));
+  HIP_TRY(dc.alloc((size_t)n * w));
+  HIP_TRY(ddm.alloc((size_t)n * w));
+  HIP_TRY(ddp.alloc((size_t)n * w));
+  HIP_TRY(hipMemcpy(dl.p, QL, sizeof(double) * n * w, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dr.p, QR, sizeof(double) * n * w, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(da.p, aux, sizeof(double) * n * aw, hipMemcpyHostToDevice));
+  unsigned nb = (unsigned)((n + 63) / 64);
+  if (nph == 1 && flux == HES_FLUX_RUSANOV)
+    hipLaunchKernelGGL((k_face<1, FLUX_RUSANOV>), dim3(nb), dim3(64), 0, 0, dl.p, dr.p, da.p, dc.p, ddm.p, ddp.p, n, m0, m1);
+  else if (nph == 1)
+    hipLaunchKernelGGL((k_face<1, FLUX_HLL>), dim3(nb), dim3(64), 0, 0, dl.p, dr.p, da.p, dc.p, ddm.p, ddp.p, n, m0, m1);
+  else if (flux == HES_FLUX_RUSANOV)
+    hipLaunchKernelGGL((k_face<2, FLUX_RUSANOV>), dim3(nb), dim3(64), 0, 0, dl.p, dr.p, da.p, dc.p, ddm.p, ddp.p, n, m0, m1);
+  else
+    hipLaunchKernelGGL((k_face<2, FLUX_HLL>), dim3(nb), dim3(64), 0, 0, dl.p, dr.p, da.p, dc.p, ddm.p, ddp.p, n, m0, m1);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(cons, dc.p, sizeof(double) * n * w, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(dm, ddm.p, sizeof(double) * n * w, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(dp, ddp.p, sizeof(double) * n * w, hipMemcpyDeviceToHost));
+  return HES_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+/* hes.h -- C-ABI of libhes.so, the gfx950 finite-volume time step of the
+ * Barton-2009 (two-phase Savenkov-Alekseev) hyperelastic solver.
+ *
+ * Conventions
+ *   - Every function returns an int status: HES_OK (0) or a negative HES_ERR_*;
+ *     hes_last_error() gives the message of the calling thread's last failure.
+ *     HES_ERR_DOMAIN corresponds to the Julia DomainError the reference raises
+ *     for sqrt of a negative det(rho A) (HyperelasticityMPh.jl:114).
+ *   - Arrays are FP64 in the reference's Julia layout: a set of n states is the
+ *     column-major Matrix (ncomp, n), i.e. component fastest.  ncomp is 14 for
+ *     one phase (the phase block without the volume fraction:
+ *     [rho, rho u1, rho u2, rho u3, rho E, rho A11, rho A21, rho A31, rho A12,
+ *     ..., rho A33]) and 30 for two phases
+ *     (HyperelasticityMPh.jl:80-84: [a, a rho, a rho u(3), a rho E, a rho vec(A)]
+ *     per phase).  3x3 tensors are column-major 9-vectors (Julia vec()).
+ *   - Materials are `nphase` consecutive blocks of 8 doubles:
+ *     (rho0, c0, cv, t0, b0, alpha, beta, gamma), the keyword arguments of the
+ *     Barton2009 constructor (EquationsOfState.jl:90); b0^2 and
+ *     k0 = c0^2 - 4/3 b0^2 are derived inside (EquationsOfState.jl:111-112).
+ *   - Host pointers are never retained.  A solver handle is not thread-safe;
+ *     the point-wise functions are re-entrant.
+ */
+#ifndef HES_H
+#define HES_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HES_ABI_VERSION 1
+
+#define HES_OK 0
+#define HES_ERR_ARG (-1)
+#define HES_ERR_HIP (-2)
+#define HES_ERR_DOMAIN (-3)
+
+#define HES_FLUX_RUSANOV 0 /* lxf with lambda = max(lambda_l, lambda_r), main.jl:46-53 */
+#define HES_FLUX_HLL 1     /* hll, main.jl:56-57 */
+
+typedef struct hes_solver hes_solver;
+
+const char* hes_last_error(void);
+int hes_version(void);
+int hes_device_info(char* arch_name, int len, int* compute_units, double* mem_gb);
+
+/* ---- Solver: replaces the time loop of main.jl:202-241 together with the
+ *      per-cell update_cell (main.jl:43-60), the CFL reduction (main.jl:204-212)
+ *      and NumFluxes.lxf / hll called per face.  The state lives on the GPU as
+ *      structure-of-arrays FP64; cells 1 and n are frozen (main.jl:219-220). */
+
+/* Allocates device state for `ncells` cells.  dx = X / nx (main.jl:147). */
+int hes_solver_create(hes_solver** out, int nphase, int flux, long long ncells,
+                      const double* materials, double cfl, double dx);
+int hes_solver_destroy(hes_solver* h);
+
+/* Upload Q0 (ncomp x ncells, Julia layout) and set the time to t0; computes the
+ * sound speeds and the CFL maximum of the state (main.jl:179, 204-212). */
+int hes_solver_set_state(hes_solver* h, const double* Q_host, double t0);
+int hes_solver_set_state_device(hes_solver* h, const double* Q_device, double t0);
+/* Piecewise-constant state built on the device: `nstates` states (ncomp x
+ * nstates) separated by nstates-1 sorted interfaces in (0,1); cell i (0-based,
+ * centre (i+1/2)/ncells) takes state #{j : centre > interface_j}, which for one
+ * interface at 1/2 is initial_condition (main.jl:99-106). */
+int hes_solver_set_piecewise(hes_solver* h, int nstates, const double* states,
+                             const double* interfaces, double t0);
+/* Download the current state (Julia layout). */
+int hes_solver_get_state(hes_solver* h, double* Q_host);
+/* Per-phase sound speed max_k sqrt|eig_k(Omega)| of every cell, (nphase, ncells). */
+int hes_solver_get_speeds(hes_solver* h, double* c_host);
+/* Time and number of steps taken since the state was set. */
+int hes_solver_time(hes_solver* h, double* t, long long* steps);
+/* dt = cfl dx / max_i lambda_i of the current state (main.jl:212). */
+int hes_solver_next_dt(hes_solver* h, double* dt);
+
+/* Advance while steps < max_steps and t < t_end (main.jl:202 `while t < T`;
+ * pass INFINITY for a fixed step count).  use_graph replays the launches of
+ * each 60-step chunk from a captured hipGraph.  Outputs may be NULL:
+ * steps_done, t_now, elapsed_ms (HIP events on the solver stream around the
+ * whole run) and dt_out (max_steps doubles: the dt of every executed step). */
+int hes_solver_run(hes_solver* h, long long max_steps, double t_end, int use_graph,
+                   long long* steps_done, double* t_now, float* elapsed_ms, double* dt_out);
+
+/* Time `reps` back-to-back launches of the step kernel (and nothing else) with
+ * HIP events on the solver stream; the state advances by reps steps. */
+int hes_solver_time_steps(hes_solver* h, long long reps, float* elapsed_ms);
+
+/* ---- Point-wise functions over n states (host arrays, synchronous). ---- */
+
+/* HyperelasticityMPh.cons2prim_mph (HyperelasticityMPh.jl:99-133):
+ * P = [a, rho, u(3), S, vec(A)] per phase (without a for one phase). */
+int hes_cons2prim(int nphase, const double* materials, const double* Q, double* P, long long n);
+/* HyperelasticityMPh.prim2cons_mph (HyperelasticityMPh.jl:63-90). */
+int hes_prim2cons(int nphase, const double* materials, const double* P, double* Q, long long n);
+/* HyperelasticityMPh.flux_mph (HyperelasticityMPh.jl:140-175). */
+int hes_flux(int nphase, const double* materials, const double* Q, double* F, long long n);
+/* HyperelasticityMPh.get_eigvals(eos, Q, nvec) (HyperelasticityMPh.jl:252-266), one
+ * direction nvec (3 doubles; NULL means [1,0,0]) for all states: per phase
+ * [u.n + c_k (k=1..3), u.n - c_k], c_k = sqrt|eig_k| in descending eig order. */
+int hes_get_eigvals(int nphase, const double* materials, const double* Q, const double* nvec,
+                    double* eig, long long n);
+/* maximum(abs.(get_eigvals(eos, Q, [1,0,0]))) per state (main.jl:209-210). */
+int hes_max_speed(int nphase, const double* materials, const double* Q, double* lambda,
+                  long long n);
+/* HyperelasticityMPh.noncons_flux (HyperelasticityMPh.jl:178-250), two phases:
+ * dense 30x30 column-major per state (n x 900 doubles). */
+int hes_noncons_flux(const double* materials, const double* Q, double* B, long long n);
+
+/* NumFluxes.lxf (NumFluxes.jl:25-60; aux = lambda per face) and NumFluxes.hll
+ * (NumFluxes.jl:70-132; aux = [minimum(eigvals[1]), maximum(eigvals[2])] per
+ * face) for n faces (QL, QR: ncomp x n).  Outputs are the (cons, dm, dp)
+ * triple the Julia functions return. */
+int hes_face_flux(int nphase, int flux, const double* materials, const double* QL,
+                  const double* QR, const double* aux, double* cons, double* dm, double* dp,
+                  long long n);
+
+/* EquationsOfState (single material, 8 doubles) */
+/* energy(eos, S, G) (EquationsOfState.jl:118-137); G: 9 x n */
+int hes_energy(const double* material, const double* S, const double* G, double* e, long long n);
+/* entropy(eos, e_int, G) (EquationsOfState.jl:139-156) */
+int hes_entropy(const double* material, const double* e_int, const double* G, double* S,
+                long long n);
+/* ForwardDiff.derivative(S -> energy(eos, S, G), S) (HyperelasticityMPh.jl:212) */
+int hes_temperature(const double* material, const double* S, const double* G, double* T,
+                    long long n);
+/* stress(eos, ent, F) (EquationsOfState.jl:179-190); sig: 9 x n column-major */
+int hes_stress(const double* material, const double* ent, const double* F, double* sig,
+               long long n);
+/* stress(eos, den, e_int, F) (EquationsOfState.jl:162-176) */
+int hes_stress_den(const double* material, const double* den, const double* e_int,
+                   const double* F, double* sig, long long n);
+/* acoustic(eos, ent, F, n) (EquationsOfState.jl:223-246); nvec: 3 x n */
+int hes_acoustic(const double* material, const double* ent, const double* F, const double* nvec,
+                 double* ac, long long n);
+/* acoustic(eos, P, n) (EquationsOfState.jl:192-221); P: 15 x n phase primitives */
+int hes_acoustic_prim(const double* material, const double* P, const double* nvec, double* ac,
+                      long long n);
+
+/* Strains (Strains.jl:26-52) */
+int hes_finger(const double* A, double* G, long long n);
+int hes_invariants(const double* G, double* inv, long long n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HES_H */

@@ -1,0 +1,145 @@
+"""GPU time step vs the oracle (HIP path through the C-ABI).
+
+Parity bar: 1e-10 relative per conserved component (max |gpu - oracle| over
+cells divided by max |oracle| of that component), the FP64 tolerance the
+reference path is held to; the time step sequences agree to the same bound.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_bind
+from oracle_bind import rel_err_per_component
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+MAT_CU = np.array([8.93, 4.6, 3.9e-4, 300.0, 2.1, 1.0, 3.0, 2.0])     # Barton2009() defaults
+MAT_AL = np.array([2.7, 4.6, 3.9e-4, 300.0, 3.1, 1.0, 3.0, 2.0])      # second phase of the 2-phase case
+
+
+def _rand_phase(rng, mat, k):
+    P = np.zeros((14, k), order="F")
+    for j in range(k):
+        A = np.eye(3) + 0.05 * rng.standard_normal((3, 3))
+        P[0, j] = mat[0] / np.linalg.det(A)
+        P[1:4, j] = rng.uniform(-0.5, 0.5, 3)
+        P[4, j] = 1e-3
+        P[5:, j] = A.reshape(-1, order="F")
+    return P
+
+
+def _rand_two_phase(rng, mats, k):
+    P = np.zeros((30, k), order="F")
+    for j in range(k):
+        a = rng.uniform(0.01, 0.99)
+        for p, al in enumerate((a, 1.0 - a)):
+            P[15 * p, j] = al
+            P[15 * p + 1:15 * p + 15, j] = _rand_phase(rng, mats[8 * p:8 * p + 8], 1)[:, 0]
+    return P
+
+
+def _piecewise(nx, Qs, ifc):
+    x = (np.arange(nx) + 0.5) / nx
+    return np.asfortranarray(Qs[:, np.searchsorted(ifc, x, side="left")])
+
+
+def _problem(ora, nph, nx, nstates, seed):
+    rng = np.random.default_rng(seed)
+    if nph == 1:
+        mats = MAT_CU.copy()
+        Ps = _rand_phase(rng, mats, nstates)
+    else:
+        mats = np.concatenate([MAT_CU, MAT_AL])
+        Ps = _rand_two_phase(rng, mats, nstates)
+    Qs = ora.prim2cons(nph, mats, Ps)
+    ifc = np.sort(rng.uniform(0.05, 0.95, nstates - 1))
+    return mats, _piecewise(nx, Qs, ifc), Qs, ifc
+
+
+def _gpu_run(nph, mats, flux, Q0, steps, use_graph=False, t_end=math.inf):
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    from hyperelasticsolver_amd.solver import Solver
+
+    def eos_of(m):
+        return Barton2009(_rho0=m[0], _c0=m[1], _cv=m[2], _t0=m[3], _b0=m[4], _alpha=m[5],
+                          _beta=m[6], _gamma=m[7])
+    eos = eos_of(mats) if nph == 1 else (eos_of(mats[:8]), eos_of(mats[8:]))
+    nx = Q0.shape[1]
+    s = Solver(eos, nx, flux=flux, cfl=0.6)
+    s.set_state(Q0)
+    r = s.run(steps=steps, t_end=t_end, use_graph=use_graph, record_dt=not use_graph)
+    Q = s.get_state()
+    s.close()
+    return Q, r
+
+
+@pytest.mark.parametrize("nph,flux,nx,steps,nstates", [
+    (1, "rusanov", 300, 60, 4),
+    (1, "hll", 300, 60, 4),
+    (2, "rusanov", 64, 12, 3),
+    (2, "hll", 64, 12, 3),
+])
+def test_step_parity(ora, nph, flux, nx, steps, nstates):
+    print(f"[parity] nph={nph} flux={flux} nx={nx} steps={steps}", flush=True)
+    mats, Q0, _, _ = _problem(ora, nph, nx, nstates, seed=100 + nph)
+    fk = 0 if flux == "rusanov" else 1
+    Qo, to, so, dto = ora.run(nph, mats, fk, Q0, 0.6, 1.0 / nx, steps=steps)
+    Qg, r = _gpu_run(nph, mats, flux, Q0, steps)
+    assert r["steps"] == so == steps
+    err = rel_err_per_component(Qg, Qo)
+    assert np.all(err <= TOL), f"per-component rel err {err}"
+    np.testing.assert_allclose(r["dt"], dto, rtol=TOL, atol=0)
+    assert abs(r["t"] - to) <= TOL * to
+
+
+def test_graph_equals_eager(ora):
+    """The hipGraph replay path gives bit-identical states to eager launches."""
+    print("[graph] single-phase hll nx=500 steps=125", flush=True)
+    mats, Q0, _, _ = _problem(ora, 1, 500, 5, seed=7)
+    Qa, ra = _gpu_run(1, mats, "hll", Q0, 125, use_graph=False)
+    Qb, rb = _gpu_run(1, mats, "hll", Q0, 125, use_graph=True)
+    assert ra["steps"] == rb["steps"] == 125
+    assert np.array_equal(Qa, Qb)
+    assert ra["t"] == rb["t"]
+
+
+def test_t_end_stops_like_reference(ora):
+    """`while t < T` (main.jl:202): same number of steps and same final time."""
+    print("[t_end] single-phase rusanov nx=200 T=0.01", flush=True)
+    mats, Q0, _, _ = _problem(ora, 1, 200, 2, seed=11)
+    Qo, to, so, _ = ora.run(1, mats, 0, Q0, 0.6, 1.0 / 200, t_end=0.01)
+    for g in (False, True):
+        Qg, r = _gpu_run(1, mats, "rusanov", Q0, None, use_graph=g, t_end=0.01)
+        assert r["steps"] == so
+        assert abs(r["t"] - to) <= 1e-12
+        assert np.all(rel_err_per_component(Qg, Qo) <= TOL)
+
+
+@pytest.mark.parametrize("nph", [1, 2])
+def test_uniform_state_is_bit_invariant(ora, nph):
+    """F(Q) - F(Q) = 0 exactly: a uniform state never changes (SURVEY known answer)."""
+    print(f"[uniform] nph={nph}", flush=True)
+    mats, _, Qs, _ = _problem(ora, nph, 10, 1, seed=3)
+    Q0 = np.asfortranarray(np.repeat(Qs[:, :1], 130, axis=1))
+    for flux in ("rusanov", "hll"):
+        Qg, r = _gpu_run(nph, mats, flux, Q0, 7)
+        assert r["steps"] == 7
+        assert np.array_equal(Qg, Q0)
+
+
+def test_conservation_telescopes(ora):
+    """Mass, momentum and energy change only by the frozen end cells' fluxes:
+    with identical end states the totals are conserved to round-off."""
+    print("[conservation] single-phase hll", flush=True)
+    mats, _, Qs, _ = _problem(ora, 1, 10, 3, seed=5)
+    nx = 400
+    Q0 = _piecewise(nx, Qs[:, [0, 1, 0]], np.array([0.4, 0.6]))
+    for flux in ("rusanov", "hll"):
+        Qg, r = _gpu_run(1, mats, flux, Q0, 50)
+        tot0 = Q0.sum(axis=1)
+        tot1 = Qg.sum(axis=1)
+        scale = np.abs(Q0).sum(axis=1)
+        assert np.all(np.abs(tot1 - tot0) <= 1e-13 * scale + 1e-300), (tot1 - tot0) / scale
