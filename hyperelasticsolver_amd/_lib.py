@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG, "libhes.so")
+# HES_LIB selects another build of the library (e.g. a variant under test)
+LIB_PATH = os.environ.get("HES_LIB") or os.path.join(_PKG, "libhes.so")
 
 HES_OK = 0
 HES_ERR_ARG = -1
@@ -56,6 +57,7 @@ _SIGS = {
     "hes_solver_run": [_vp, _ll, _d, _i, ctypes.POINTER(_ll), _dp,
                        ctypes.POINTER(ctypes.c_float), _dp],
     "hes_solver_time_steps": [_vp, _ll, ctypes.POINTER(ctypes.c_float)],
+    "hes_copy_bandwidth": [_ll, _i, _dp],
     "hes_cons2prim": [_i, _dp, _dp, _dp, _ll],
     "hes_prim2cons": [_i, _dp, _dp, _dp, _ll],
     "hes_flux": [_i, _dp, _dp, _dp, _ll],

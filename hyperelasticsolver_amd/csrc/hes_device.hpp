@@ -57,6 +57,15 @@ struct Thermo {
   int bad;               // non-positive det(rho A) or non-finite state
 };
 
+// 1/x from the hardware reciprocal refined by two Newton steps (error within
+// about an ulp; the step does not need IEEE-rounded division).
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return r;
+}
+
 __device__ __forceinline__ double det3cm(const double* a) {
   // column-major a(i,j) = a[i + 3j]
   return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[3] * (a[1] * a[8] - a[7] * a[2]) +
@@ -66,7 +75,9 @@ __device__ __forceinline__ double det3cm(const double* a) {
 // Finger tensor, invariants and energy split of a phase.
 // q points at [a r, a r u(3), a r E, a r A(9)] (the phase block without a).
 // HyperelasticityMPh.jl:106-124 (cons2prim) + Strains.jl:26-52 + EquationsOfState.jl:139-156.
-template <bool ONE_FRAC>
+// FAST: 1 = alpha 1, beta 3, gamma 2 (no pow), 0 = general exponents,
+// -1 = decide from m.fast at run time.
+template <bool ONE_FRAC, int FAST = -1>
 __device__ __forceinline__ void thermo(const MatDev& m, double frac, const double* q, Thermo& t) {
   const double* P = q + 5;
   double detP = det3cm(P);
@@ -74,13 +85,13 @@ __device__ __forceinline__ void thermo(const MatDev& m, double frac, const doubl
   if (ONE_FRAC) {
     rho2 = detP * m.irho0;
   } else {
-    double ifr = 1.0 / frac;
+    double ifr = rcp(frac);
     rho2 = detP * (ifr * ifr * ifr) * m.irho0;
   }
   t.bad = !(detP > 0.0);
   t.rho = sqrt(rho2);
   t.den = ONE_FRAC ? t.rho : frac * t.rho;
-  double iden = 1.0 / t.den;
+  double iden = rcp(t.den);
   t.u[0] = q[1] * iden;
   t.u[1] = q[2] * iden;
   t.u[2] = q[3] * iden;
@@ -100,7 +111,7 @@ __device__ __forceinline__ void thermo(const MatDev& m, double frac, const doubl
   double C12 = B13 * B23 - B12 * B33;
   double C13 = B12 * B23 - B13 * B22;
   double detB = B11 * C11 + B12 * C12 + B13 * C13;
-  double idetB = 1.0 / detB;
+  double idetB = rcp(detB);
   t.G11 = C11 * idetB;
   t.G22 = (B11 * B33 - B13 * B13) * idetB;
   t.G33 = (B11 * B22 - B12 * B12) * idetB;
@@ -113,7 +124,7 @@ __device__ __forceinline__ void thermo(const MatDev& m, double frac, const doubl
   t.i2 = 0.5 * (t.i1 * t.i1 - trG2);
   t.i3 = idetB;
   double ith;  // 1 / (cv t0 s_g)
-  if (m.fast) {
+  if (FAST == 1 || (FAST == -1 && m.fast)) {
     // det A = rho0 / rho, so I3^(1/2) = 1/det A = rho / rho0
     t.sa = t.rho * m.irho0;
     t.sb = t.i3 * t.sa;
@@ -124,7 +135,7 @@ __device__ __forceinline__ void thermo(const MatDev& m, double frac, const doubl
     t.sa = exp(m.half_a * li3);
     t.sb = exp(m.half_b * li3);
     t.sg = exp(m.half_g * li3);
-    ith = 1.0 / (m.cvt0 * t.sg);
+    ith = rcp(m.cvt0 * t.sg);
   }
   t.W = m.hb0sq * t.sb * (t.i1 * t.i1 * (1.0 / 3.0) - t.i2);
   double sam1 = t.sa - 1.0;
@@ -316,7 +327,7 @@ __device__ __forceinline__ double cubic_top_root(double c2, double c1, double c0
     double f = ((t - c2) * t + c1) * t - c0;
     double fp = (3.0 * t - 2.0 * c2) * t + c1;
     if (!(fp > 0.0)) break;
-    double tn = t - f / fp;
+    double tn = t - f * rcp(fp);
     if (!(tn < t)) break;
     t = tn < lb ? lb : tn;
   }
@@ -332,7 +343,7 @@ __device__ __forceinline__ double cubic_bottom_root(double c2, double c1, double
     double f = ((t - c2) * t + c1) * t - c0;
     double fp = (3.0 * t - 2.0 * c2) * t + c1;
     if (!(fp > 0.0)) break;
-    double tn = t - f / fp;
+    double tn = t - f * rcp(fp);
     if (!(tn > t)) break;
     t = tn > ub ? ub : tn;
   }
@@ -348,8 +359,15 @@ __device__ __forceinline__ double spectral_radius_sym(const double O[6]) {
   double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
   double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
   double r1 = fabs(d) + fabs(e), r2 = fabs(d) + fabs(f), r3 = fabs(e) + fabs(f);
-  double ub = fmax(a + r1, fmax(b + r2, c + r3));
   double lb = fmin(a - r1, fmin(b - r2, c - r3));
+  // Upper bound of the top eigenvalue: with O = [a w^T; w D], lmax <= max(a, lD) + |w|
+  // (lD = top eigenvalue of D), and when a > lD the secular equation
+  // lmax = a + w^T (lmax - D)^-1 w with lmax >= a gives lmax <= a + |w|^2 / (a - lD).
+  double hd = 0.5 * (b - c);
+  double lD = 0.5 * (b + c) + sqrt(hd * hd + f * f);
+  double ww = d * d + e * e;
+  double ub = fmax(a, lD) + sqrt(ww);
+  if (a > lD) ub = fmin(ub, a + ww * rcp(a - lD));
   double top = cubic_top_root(c2, c1, c0, ub, lb);
   double rad = fabs(top);
   if (lb < -rad) {

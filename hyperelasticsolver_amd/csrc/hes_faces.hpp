@@ -38,20 +38,20 @@ template <int NPH>
 struct NComp { static constexpr int value = NPH == 1 ? 14 : 30; };
 
 // Thermo state of phase p of a state vector in the model layout.
-template <int NPH>
+template <int NPH, int FAST = -1>
 __device__ __forceinline__ void phase_thermo(const MatDev* mat, const double* q, int p, Thermo& t) {
-  if (NPH == 1) thermo<true>(mat[0], 1.0, q, t);
-  else thermo<false>(mat[p], q[15 * p], q + 15 * p + 1, t);
+  if (NPH == 1) thermo<true, FAST>(mat[0], 1.0, q, t);
+  else thermo<false, FAST>(mat[p], q[15 * p], q + 15 * p + 1, t);
 }
 
 // Flux F(Q) of a cell and the wave-speed bounds of its eigenvalues
 // [min_p(u1 - c_p), max_p(u1 + c_p)] from the stored sound speeds cc[p].
-template <int NPH>
+template <int NPH, int FAST = -1>
 __device__ __forceinline__ void cell_flux(const MatDev* mat, const double* q, const double* cc,
                                           double* f, double& smin, double& smax, int& bad) {
   if (NPH == 1) {
     Thermo t;
-    thermo<true>(mat[0], 1.0, q, t);
+    thermo<true, FAST>(mat[0], 1.0, q, t);
     phase_flux(mat[0], 1.0, t, f);
     smin = t.u[0] - cc[0];
     smax = t.u[0] + cc[0];
@@ -61,7 +61,7 @@ __device__ __forceinline__ void cell_flux(const MatDev* mat, const double* q, co
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       Thermo t;
-      thermo<false>(mat[p], q[15 * p], q + 15 * p + 1, t);
+      thermo<false, FAST>(mat[p], q[15 * p], q + 15 * p + 1, t);
       f[15 * p] = 0.0;
       phase_flux(mat[p], q[15 * p], t, f + 15 * p + 1);
       double lo = t.u[0] - cc[p], hi = t.u[0] + cc[p];
@@ -74,7 +74,7 @@ __device__ __forceinline__ void cell_flux(const MatDev* mat, const double* q, co
 
 // Sound speeds of a cell (per phase) and its CFL speed max_p(|u1_p| + c_p)
 // (main.jl:209-210).
-template <int NPH>
+template <int NPH, int FAST = -1>
 __device__ __forceinline__ double cell_speeds(const MatDev* mat, const double* q, double* cc,
                                               int& bad) {
   double lam = 0.0;
@@ -82,7 +82,7 @@ __device__ __forceinline__ double cell_speeds(const MatDev* mat, const double* q
 #pragma unroll
   for (int p = 0; p < NPH; ++p) {
     Thermo t;
-    phase_thermo<NPH>(mat, q, p, t);
+    phase_thermo<NPH, FAST>(mat, q, p, t);
     double c = sound_speed(mat[NPH == 1 ? 0 : p], t);
     cc[p] = c;
     lam = fmax(lam, fabs(t.u[0]) + c);
@@ -92,13 +92,13 @@ __device__ __forceinline__ double cell_speeds(const MatDev* mat, const double* q
 }
 
 // Eigenvalue range of a state from scratch (HLL mid state, NumFluxes.jl:90-91).
-template <int NPH>
+template <int NPH, int FAST = -1>
 __device__ __forceinline__ void state_speed_range(const MatDev* mat, const double* q, double& smin,
                                                   double& smax) {
 #pragma unroll
   for (int p = 0; p < NPH; ++p) {
     Thermo t;
-    phase_thermo<NPH>(mat, q, p, t);
+    phase_thermo<NPH, FAST>(mat, q, p, t);
     double c = sound_speed(mat[NPH == 1 ? 0 : p], t);
     double lo = t.u[0] - c, hi = t.u[0] + c;
     smin = p == 0 ? lo : fmin(smin, lo);
@@ -116,16 +116,17 @@ struct BCol {
   double a[2][9];   // rows 7:15 of block p
 };
 
+template <int FAST = -1>
 __device__ __forceinline__ void bcol2(const MatDev* mat, const double* q, BCol& b) {
   Thermo t0, t1;
-  thermo<false>(mat[0], q[0], q + 1, t0);
-  thermo<false>(mat[1], q[15], q + 16, t1);
+  thermo<false, FAST>(mat[0], q[0], q + 1, t0);
+  thermo<false, FAST>(mat[1], q[15], q + 16, t1);
   double s0[3], s1[3];
   stress_row1(mat[0], t0, s0);
   stress_row1(mat[1], t1, s1);
   double th0 = temperature_of(mat[0], t0), th1 = temperature_of(mat[1], t1);
   // sigma_I = (k2 th2 K1 + k1 th1 K2) / (k1 th1 + k2 th2), K_p = sigma_p
-  double iden = 1.0 / (0.5 * th0 + 0.5 * th1);
+  double iden = rcp(0.5 * th0 + 0.5 * th1);
   double w0 = 0.5 * th1 * iden, w1 = 0.5 * th0 * iden;
   double uI[3];
 #pragma unroll
@@ -153,7 +154,7 @@ __device__ __forceinline__ void bcol2(const MatDev* mat, const double* q, BCol& 
 // Integral of B(psi) dpsi/ds along psi(s) = qa (1 - s) + qb s over [0,1]
 // (NumFluxes.jl:97-107 / 39-47).  B has one non-zero column per phase block,
 // so the integral is (sum_q w_q b^p(psi_q)) times the volume-fraction jump.
-template <bool LOBATTO>
+template <bool LOBATTO, int FAST = -1>
 __device__ void bint2(const MatDev* mat, const double* qa, const double* qb, double* out) {
   double da0 = qb[0] - qa[0], da1 = qb[15] - qa[15];
   if (da0 == 0.0 && da1 == 0.0) {
@@ -174,7 +175,7 @@ __device__ void bint2(const MatDev* mat, const double* qa, const double* qb, dou
 #pragma unroll
     for (int r = 0; r < 30; ++r) psi[r] = qa[r] * oms + qb[r] * s;
     BCol b;
-    bcol2(mat, psi, b);
+    bcol2<FAST>(mat, psi, b);
     acc_u += w * b.uI0;
     acc_s0 += w * b.sI[0];
     acc_s1 += w * b.sI[1];
@@ -203,7 +204,7 @@ __device__ void bint2(const MatDev* mat, const double* qa, const double* qb, dou
 
 // Rusanov / LxF face (NumFluxes.jl:25-60): conservative flux and the common
 // half path integral D- = D+.
-template <int NPH>
+template <int NPH, int FAST = -1>
 __device__ __forceinline__ void face_rusanov(const MatDev* mat, const double* ql, const double* qr,
                                              const double* fl, const double* fr, double lam,
                                              double* cons, double* dmp) {
@@ -212,7 +213,7 @@ __device__ __forceinline__ void face_rusanov(const MatDev* mat, const double* ql
 #pragma unroll
   for (int r = 0; r < NC; ++r) cons[r] = 0.5 * (fl[r] + fr[r]) - hl * (qr[r] - ql[r]);
   if (NPH == 2) {
-    bint2<true>(mat, ql, qr, dmp);
+    bint2<true, FAST>(mat, ql, qr, dmp);
 #pragma unroll
     for (int r = 0; r < NC; ++r) dmp[r] = 0.5 * dmp[r];
   }
@@ -220,7 +221,7 @@ __device__ __forceinline__ void face_rusanov(const MatDev* mat, const double* ql
 
 // HLL face (NumFluxes.jl:70-132): D- and D+ (the conservative part is zero).
 // smin_l = min eig of the left cell, smax_r = max eig of the right cell.
-template <int NPH>
+template <int NPH, int FAST = -1>
 __device__ __forceinline__ void face_hll(const MatDev* mat, const double* ql, const double* qr,
                                          const double* fl, const double* fr, double smin_l,
                                          double smax_r, double* dm, double* dp) {
@@ -229,11 +230,12 @@ __device__ __forceinline__ void face_hll(const MatDev* mat, const double* ql, co
 #pragma unroll
   for (int r = 0; r < NC; ++r) qm[r] = 0.5 * (ql[r] + qr[r]);
   double mn_m, mx_m;
-  state_speed_range<NPH>(mat, qm, mn_m, mx_m);
+  state_speed_range<NPH, FAST>(mat, qm, mn_m, mx_m);
   double s_l = fmin(0.0, fmin(mn_m, smin_l));
   double s_r = fmax(0.0, fmax(mx_m, smax_r));
   double sd = s_r - s_l;
-  double cl = -s_l / sd, cr = s_r / sd, clr = s_l * s_r / sd;
+  double isd = rcp(sd);
+  double cl = -s_l * isd, cr = s_r * isd, clr = s_l * s_r * isd;
   if (NPH == 1) {
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
@@ -245,23 +247,23 @@ __device__ __forceinline__ void face_hll(const MatDev* mat, const double* ql, co
   } else {
     // path_int = B_int(Q_l, Q_r) + F(Q_r) - F(Q_l)
     double J[NC];
-    bint2<false>(mat, ql, qr, J);
+    bint2<false, FAST>(mat, ql, qr, J);
 #pragma unroll
     for (int r = 0; r < NC; ++r) J[r] = J[r] + (fr[r] - fl[r]);
     // Q_hll; the volume-fraction rows are written as a_l + (s_r (a_r - a_l) - P)/(s_r - s_l),
     // algebraically the same and exact when a_l == a_r with no path jump.
     double qh[NC];
 #pragma unroll
-    for (int r = 0; r < NC; ++r) qh[r] = (qr[r] * s_r - ql[r] * s_l - J[r]) / sd;
-    qh[0] = ql[0] + (s_r * (qr[0] - ql[0]) - J[0]) / sd;
-    qh[15] = ql[15] + (s_r * (qr[15] - ql[15]) - J[15]) / sd;
+    for (int r = 0; r < NC; ++r) qh[r] = (qr[r] * s_r - ql[r] * s_l - J[r]) * isd;
+    qh[0] = ql[0] + (s_r * (qr[0] - ql[0]) - J[0]) * isd;
+    qh[15] = ql[15] + (s_r * (qr[15] - ql[15]) - J[15]) * isd;
     double b[NC];
 #pragma unroll
     for (int r = 0; r < NC; ++r) J[r] = fr[r] - fl[r];
-    bint2<false>(mat, ql, qh, b);
+    bint2<false, FAST>(mat, ql, qh, b);
 #pragma unroll
     for (int r = 0; r < NC; ++r) J[r] = J[r] + b[r];
-    bint2<false>(mat, qh, qr, b);
+    bint2<false, FAST>(mat, qh, qr, b);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
       double Jr = J[r] + b[r];

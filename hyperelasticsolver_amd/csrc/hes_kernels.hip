@@ -45,7 +45,10 @@ int fail(int code, const char* msg) {
   } while (0)
 
 constexpr int TILE = 62;      // owned cells per wave
-constexpr int BLOCK = 256;    // 4 waves
+#ifndef HES_STEP_BLOCK
+#define HES_STEP_BLOCK 256
+#endif
+constexpr int BLOCK = HES_STEP_BLOCK;  // threads per block of the step kernel
 
 MatDev make_mat(const double* p) {
   MatDev m;
@@ -108,15 +111,20 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 __device__ __forceinline__ void block_max_to_slot(double v, unsigned long long* slot) {
-  __shared__ double red[BLOCK / 64];
-  v = wave_max(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = red[0];
+  double m;
+  if (BLOCK == 64) {
+    m = wave_max(v);
+  } else {
+    __shared__ double red[BLOCK / 64];
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    m = red[0];
 #pragma unroll
     for (int k = 1; k < BLOCK / 64; ++k) m = fmax(m, red[k]);
+  }
+  if (threadIdx.x == 0) {
     if (m > 0.0) {
       unsigned long long bits = (unsigned long long)__double_as_longlong(m);
       unsigned long long cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -125,35 +133,61 @@ __device__ __forceinline__ void block_max_to_slot(double v, unsigned long long* 
   }
 }
 
-template <int NPH, int FLUX>
-__global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
+// Streaming access to the state: each element is read once and written once
+// per step, so it bypasses cache allocation.
+#ifndef HES_NT_LOAD
+#define HES_NT_LOAD 1
+#endif
+#ifndef HES_NT_STORE
+#define HES_NT_STORE 0
+#endif
+__device__ __forceinline__ double ld_stream(const double* p) {
+#if HES_NT_LOAD
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ void st_stream(double v, double* p) {
+#if HES_NT_STORE
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
+// Block order across the 8 XCDs: the hardware deals consecutive blocks to
+// XCDs round-robin; remap so each XCD walks a contiguous stretch of cells and
+// the cache lines two neighbouring tiles share stay in one L2.
+__device__ __forceinline__ long long xcd_block() {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned q = nb / 8;
+  if (b < 8 * q) return (long long)(b % 8) * q + b / 8;
+  return b;
+}
+
+#ifndef HES_STEP_WAVES_PER_EU
+#define HES_STEP_WAVES_PER_EU 3
+#endif
+#ifndef HES_STRIP
+#define HES_STRIP 0  // chunks per wave in the strip kernel (0: one chunk per wave)
+#endif
+
+// One 64-lane chunk: lane l holds cell i = first + l - 1 (lanes 0 and 63 are
+// halo cells); q and cc are its state and sound speeds.  Updates the owned
+// interior cells, writes their new state and sound speeds, and returns the
+// lane's CFL speed of the new state (0 for lanes that own nothing).
+template <int NPH, int FLUX, int FAST>
+__device__ __forceinline__ double chunk_step(const StepArgs& a, long long i, const double* q,
+                                             const double* cc, double lmax) {
   constexpr int NC = NComp<NPH>::value;
   const int lane = threadIdx.x & 63;
-  const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
-  const long long i = wave * TILE + lane - 1;
   const long long n = a.n, pitch = a.pitch;
-
-  const double tcur = a.tring[a.step & 1];
-  if (!(tcur < a.t_end)) {  // main.jl:202 `while t < T`
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.tring[(a.step + 1) & 1] = tcur;
-    return;
-  }
-  const double lmax = __longlong_as_double(
-      (long long)__hip_atomic_load(a.slots + (a.step % 3), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  const double dt = a.cfl * a.dx / lmax;  // main.jl:212
-  const double dtdx = dt / a.dx;
-
   const bool inr = (i >= 0) && (i < n);
-  const long long ic = i < 0 ? 0 : (i >= n ? n - 1 : i);
-  double q[NC], cc[NPH];
-#pragma unroll
-  for (int r = 0; r < NC; ++r) q[r] = a.Qin[r * pitch + ic];
-#pragma unroll
-  for (int p = 0; p < NPH; ++p) cc[p] = a.Cin[p * pitch + ic];
 
   double f[NC], smin, smax;
   int bad;
-  cell_flux<NPH>(a.mat, q, cc, f, smin, smax, bad);
+  cell_flux<NPH, FAST>(a.mat, q, cc, f, smin, smax, bad);
 
   // right neighbour (cell i+1) from lane+1
   double qn[NC], fn[NC];
@@ -164,13 +198,17 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
   }
   double smin_n = __shfl_down(smin, 1), smax_n = __shfl_down(smax, 1);
 
+  // dt is needed only from here on: the CFL maximum is read after the state
+  // loads are in flight (main.jl:212, dt/dx of main.jl:225)
+  const double dtdx = (a.cfl * a.dx / lmax) / a.dx;
+
   // face i+1/2 (this lane) and face i-1/2 (from lane-1)
   double upd[NC];
   if (FLUX == FLUX_RUSANOV) {
     double lam_c = fmax(fabs(smin), fabs(smax));       // maximum(abs.(eigvals))
     double lam_n = fmax(fabs(smin_n), fabs(smax_n));
     double cons[NC], dmp[NC];
-    face_rusanov<NPH>(a.mat, q, qn, f, fn, fmax(lam_c, lam_n), cons, dmp);
+    face_rusanov<NPH, FAST>(a.mat, q, qn, f, fn, fmax(lam_c, lam_n), cons, dmp);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
       double cl = __shfl_up(cons[r], 1);
@@ -180,7 +218,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
     }
   } else {
     double dm[NC], dp[NC];
-    face_hll<NPH>(a.mat, q, qn, f, fn, smin, smax_n, dm, dp);
+    face_hll<NPH, FAST>(a.mat, q, qn, f, fn, smin, smax_n, dm, dp);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
       double dpl = __shfl_up(dp[r], 1);
@@ -194,26 +232,106 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
   int bad2 = 0;
   if (interior) {
 #pragma unroll
-    for (int r = 0; r < NC; ++r) a.Qout[r * pitch + i] = upd[r];
+    for (int r = 0; r < NC; ++r) st_stream(upd[r], a.Qout + r * pitch + i);
     double cn[NPH];
-    lam_new = cell_speeds<NPH>(a.mat, upd, cn, bad2);
+    lam_new = cell_speeds<NPH, FAST>(a.mat, upd, cn, bad2);
 #pragma unroll
-    for (int p = 0; p < NPH; ++p) a.Cout[p * pitch + i] = cn[p];
+    for (int p = 0; p < NPH; ++p) st_stream(cn[p], a.Cout + p * pitch + i);
   } else if (owned) {  // frozen end cells (main.jl:219-220)
     lam_new = fmax(fabs(smin), fabs(smax));
 #pragma unroll
-    for (int p = 0; p < NPH; ++p) a.Cout[p * pitch + i] = cc[p];
+    for (int p = 0; p < NPH; ++p) st_stream(cc[p], a.Cout + p * pitch + i);
   }
   int anybad = (owned && bad) || bad2;
   if (__any(anybad) && lane == 0) atomicOr(a.status, 1);
+  return lam_new;
+}
 
+template <int NPH>
+__device__ __forceinline__ void load_chunk(const StepArgs& a, long long i, double* q, double* cc) {
+  constexpr int NC = NComp<NPH>::value;
+  const long long ic = i < 0 ? 0 : (i >= a.n ? a.n - 1 : i);
+#pragma unroll
+  for (int r = 0; r < NC; ++r) q[r] = ld_stream(a.Qin + r * a.pitch + ic);
+#pragma unroll
+  for (int p = 0; p < NPH; ++p) cc[p] = ld_stream(a.Cin + p * a.pitch + ic);
+}
+
+// Time bookkeeping of a step: returns false if t >= t_end (no step).
+__device__ __forceinline__ bool step_begin(const StepArgs& a, double& tcur) {
+  tcur = a.tring[a.step & 1];
+  if (!(tcur < a.t_end)) {  // main.jl:202 `while t < T`
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.tring[(a.step + 1) & 1] = tcur;
+    return false;
+  }
+  return true;
+}
+
+// max lambda of the current state, max-reduced by the previous launch (a
+// plain load: the previous kernel's atomics are visible at launch, and a plain
+// load lets the compiler issue it together with the state loads).
+__device__ __forceinline__ double step_lmax(const StepArgs& a) {
+  return __longlong_as_double(*(const long long*)(a.slots + (a.step % 3)));
+}
+
+__device__ __forceinline__ void step_epilogue(const StepArgs& a, double lam_new, double tcur,
+                                              double lmax) {
   block_max_to_slot(lam_new, a.slots + ((a.step + 1) % 3));
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double dt = a.cfl * a.dx / lmax;  // main.jl:212
     a.tring[(a.step + 1) & 1] = tcur + dt;
     a.slots[(a.step + 2) % 3] = 0ull;
     if (a.dt_hist) a.dt_hist[a.hist_idx] = dt;
     atomicAdd(a.status + 1, 1);
   }
+}
+
+// One chunk per wave.
+template <int NPH, int FLUX, int FAST>
+__global__ __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : 1))) void k_step(StepArgs a) {
+  constexpr int NC = NComp<NPH>::value;
+  const int lane = threadIdx.x & 63;
+  const long long wave = (xcd_block() * BLOCK + threadIdx.x) >> 6;
+  const long long i = wave * TILE + lane - 1;
+  double tcur;
+  if (!step_begin(a, tcur)) return;
+  double q[NC], cc[NPH];
+  load_chunk<NPH>(a, i, q, cc);
+  const double lmax = step_lmax(a);
+  double lam = chunk_step<NPH, FLUX, FAST>(a, i, q, cc, lmax);
+  step_epilogue(a, lam, tcur, lmax);
+}
+
+// A strip of consecutive chunks per wave; the next chunk's state is loaded
+// while the current chunk computes.
+template <int NPH, int FLUX, int FAST>
+__global__ __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : 1))) void k_step_strip(
+    StepArgs a, int chunks_per_wave) {
+  constexpr int NC = NComp<NPH>::value;
+  const int lane = threadIdx.x & 63;
+  const long long wave = (xcd_block() * BLOCK + threadIdx.x) >> 6;
+  const long long nchunks = (a.n + TILE - 1) / TILE;
+  long long c = wave * chunks_per_wave;
+  const long long cend = c + chunks_per_wave < nchunks ? c + chunks_per_wave : nchunks;
+  double tcur;
+  if (!step_begin(a, tcur)) return;
+  double lam = 0.0;
+  double q[NC], cc[NPH];
+  if (c < cend) load_chunk<NPH>(a, c * TILE + lane - 1, q, cc);
+  const double lmax = step_lmax(a);
+#pragma unroll 1
+  for (; c < cend; ++c) {
+    double qn[NC], ccn[NPH];
+    if (c + 1 < cend) load_chunk<NPH>(a, (c + 1) * TILE + lane - 1, qn, ccn);
+    lam = fmax(lam, chunk_step<NPH, FLUX, FAST>(a, c * TILE + lane - 1, q, cc, lmax));
+#pragma unroll
+    for (int r = 0; r < NC; ++r) q[r] = qn[r];
+#pragma unroll
+    for (int p = 0; p < NPH; ++p) cc[p] = ccn[p];
+  }
+  step_epilogue(a, lam, tcur, lmax);
 }
 
 // Sound speeds of every cell of a state and the max CFL speed (main.jl:204-212).
@@ -521,17 +639,38 @@ struct hes_solver {
 namespace {
 
 template <int NPH, int FLUX>
-void launch_step_t(const StepArgs& a, long long nwaves, hipStream_t s) {
-  long long nblocks = (nwaves + 3) / 4;
-  hipLaunchKernelGGL((k_step<NPH, FLUX>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
+void launch_step_t(const StepArgs& a, long long nchunks, bool fast, hipStream_t s) {
+  constexpr int WPB = BLOCK / 64;
+  const int K = HES_STRIP;
+  if (K > 0) {
+    long long nwaves = (nchunks + K - 1) / K;
+    long long nblocks = (nwaves + WPB - 1) / WPB;
+    if (fast)
+      hipLaunchKernelGGL((k_step_strip<NPH, FLUX, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a, K);
+    else
+      hipLaunchKernelGGL((k_step_strip<NPH, FLUX, 0>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a, K);
+    return;
+  }
+  long long nblocks = (nchunks + WPB - 1) / WPB;
+  if (fast)
+    hipLaunchKernelGGL((k_step<NPH, FLUX, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_step<NPH, FLUX, 0>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
 }
+
+bool all_fast(const hes_solver* h);
 
 void launch_step(hes_solver* h, const StepArgs& a) {
   long long nwaves = (h->n + TILE - 1) / TILE;
-  if (h->nph == 1 && h->flux == FLUX_RUSANOV) launch_step_t<1, FLUX_RUSANOV>(a, nwaves, h->stream);
-  else if (h->nph == 1) launch_step_t<1, FLUX_HLL>(a, nwaves, h->stream);
-  else if (h->flux == FLUX_RUSANOV) launch_step_t<2, FLUX_RUSANOV>(a, nwaves, h->stream);
-  else launch_step_t<2, FLUX_HLL>(a, nwaves, h->stream);
+  bool fast = all_fast(h);
+  if (h->nph == 1 && h->flux == FLUX_RUSANOV) launch_step_t<1, FLUX_RUSANOV>(a, nwaves, fast, h->stream);
+  else if (h->nph == 1) launch_step_t<1, FLUX_HLL>(a, nwaves, fast, h->stream);
+  else if (h->flux == FLUX_RUSANOV) launch_step_t<2, FLUX_RUSANOV>(a, nwaves, fast, h->stream);
+  else launch_step_t<2, FLUX_HLL>(a, nwaves, fast, h->stream);
+}
+
+bool all_fast(const hes_solver* h) {
+  return h->mat[0].fast && (h->nph == 1 || h->mat[1].fast);
 }
 
 StepArgs make_args(hes_solver* h, long long s, double t_end, double* hist, int hist_idx) {
@@ -885,6 +1024,45 @@ int hes_solver_time_steps(hes_solver* h, long long reps, float* elapsed_ms) {
   HIP_TRY(hipMemcpy(tr, h->tring, sizeof(tr), hipMemcpyDeviceToHost));
   h->t = tr[h->step & 1];
   if (st[0]) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
+  return HES_OK;
+}
+
+// Device-to-device copy bandwidth (bytes read + written per second) of a plain
+// streaming copy of `bytes` bytes, best of `reps`: the measured second
+// denominator of the HBM roofline.
+__global__ void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+int hes_copy_bandwidth(long long bytes, int reps, double* gbs) {
+  if (!gbs || bytes < 1024 || reps < 1) return fail(HES_ERR_ARG, "bad argument");
+  long long n = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  HIP_TRY(hipMalloc(&a, n * 16));
+  hipError_t e = hipMalloc(&b, n * 16);
+  hipEvent_t e0, e1;
+  if (e == hipSuccess) e = hipMemset(a, 0, n * 16);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  float best = 1e30f;
+  for (int r = 0; r <= reps && e == hipSuccess; ++r) {
+    e = hipEventRecord(e0, 0);
+    hipLaunchKernelGGL(k_copy, dim3(256 * 64), dim3(256), 0, 0, a, b, n);
+    if (e == hipSuccess) e = hipEventRecord(e1, 0);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (r > 0 && ms < best) best = ms;  // first launch is a warm-up
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) {
+    g_err = std::string("copy bandwidth: ") + hipGetErrorString(e);
+    return HES_ERR_HIP;
+  }
+  *gbs = 2.0 * (double)n * 16.0 / (best * 1e-3) / 1e9;
   return HES_OK;
 }
 

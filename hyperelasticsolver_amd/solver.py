@@ -169,3 +169,10 @@ def device_info():
     mem = ctypes.c_double()
     check(lib().hes_device_info(name, 64, ctypes.byref(cus), ctypes.byref(mem)))
     return {"arch": name.value.decode(), "compute_units": cus.value, "mem_gb": mem.value}
+
+
+def copy_bandwidth(nbytes=1 << 31, reps=5):
+    """Measured device-copy bandwidth in GB/s (read + write bytes per second)."""
+    g = ctypes.c_double()
+    check(lib().hes_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(g)))
+    return g.value

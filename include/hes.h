@@ -84,6 +84,11 @@ int hes_solver_run(hes_solver* h, long long max_steps, double t_end, int use_gra
  * HIP events on the solver stream; the state advances by reps steps. */
 int hes_solver_time_steps(hes_solver* h, long long reps, float* elapsed_ms);
 
+/* Measured HBM bandwidth of a plain device copy of `bytes` bytes (read plus
+ * written bytes per second, GB/s, best of `reps`): the second roofline
+ * denominator beside the 8 TB/s specification.  No counterpart in the reference. */
+int hes_copy_bandwidth(long long bytes, int reps, double* gbs);
+
 /* ---- Point-wise functions over n states (host arrays, synchronous). ---- */
 
 /* HyperelasticityMPh.cons2prim_mph (HyperelasticityMPh.jl:99-133):

@@ -145,3 +145,19 @@ def test_conservation_telescopes(ora):
         tot1 = Qg.sum(axis=1)
         scale = np.abs(Q0).sum(axis=1)
         assert np.all(np.abs(tot1 - tot0) <= 1e-13 * scale + 1e-300), (tot1 - tot0) / scale
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "hll"])
+def test_parity_many_blocks(ora, flux):
+    """2^16 cells: ~1000 blocks spread over all XCDs, so the CFL max-reduction
+    across blocks and the XCD block order are exercised."""
+    nx, steps = 1 << 16, 6
+    print(f"[parity-large] flux={flux} nx={nx} steps={steps}", flush=True)
+    mats, Q0, _, _ = _problem(ora, 1, nx, 33, seed=404)
+    fk = 0 if flux == "rusanov" else 1
+    Qo, to, so, dto = ora.run(1, mats, fk, Q0, 0.6, 1.0 / nx, steps=steps)
+    Qg, r = _gpu_run(1, mats, flux, Q0, steps)
+    err = rel_err_per_component(Qg, Qo)
+    print(f"  max per-component rel err {err.max():.3e}", flush=True)
+    assert np.all(err <= TOL), err
+    np.testing.assert_allclose(r["dt"], dto, rtol=TOL, atol=0)
