@@ -169,9 +169,6 @@ __device__ __forceinline__ long long xcd_block() {
 #ifndef HES_STEP_WAVES_PER_EU
 #define HES_STEP_WAVES_PER_EU 3
 #endif
-#ifndef HES_STRIP
-#define HES_STRIP 0  // chunks per wave in the strip kernel (0: one chunk per wave)
-#endif
 
 // One 64-lane chunk: lane l holds cell i = first + l - 1 (lanes 0 and 63 are
 // halo cells); q and cc are its state and sound speeds.  Updates the owned
@@ -189,6 +186,30 @@ __device__ __forceinline__ double chunk_step(const StepArgs& a, long long i, con
   int bad;
   cell_flux<NPH, FAST>(a.mat, q, cc, f, smin, smax, bad);
 
+  // dt is needed only from here on: the CFL maximum is read after the state
+  // loads are in flight (main.jl:212, dt/dx of main.jl:225)
+  const double dtdx = (a.cfl * a.dx / lmax) / a.dx;
+
+  double upd[NC];
+  if (FLUX == FLUX_RUSANOV && NPH == 1) {
+    // Rusanov face i+1/2: 1/2 (F_i + F_i+1) - 1/2 lam (Q_i+1 - Q_i) with
+    // lam = max(lam_i, lam_i+1) (NumFluxes.jl:30, main.jl:47-48), split as
+    // a_i + b_i+1 with a = F/2 + lam/2 Q and b = F/2 - lam/2 Q.  Each face sum
+    // is formed from the same two rounded halves by both of its cells, so the
+    // two cells see bit-identical face fluxes.
+    double lam_c = fmax(fabs(smin), fabs(smax));  // maximum(abs.(eigvals))
+    double hr = 0.5 * fmax(lam_c, __shfl_down(lam_c, 1));
+    double hl = 0.5 * fmax(__shfl_up(lam_c, 1), lam_c);
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+      double fa = 0.5 * f[r] + hr * q[r];
+      double fb = 0.5 * f[r] - hl * q[r];
+      double Fr = fa + __shfl_down(fb, 1);
+      double Fl = __shfl_up(fa, 1) + fb;
+      upd[r] = q[r] - dtdx * (Fr - Fl);  // main.jl:59 with zero path integrals
+    }
+  } else {
+
   // right neighbour (cell i+1) from lane+1
   double qn[NC], fn[NC];
 #pragma unroll
@@ -198,12 +219,7 @@ __device__ __forceinline__ double chunk_step(const StepArgs& a, long long i, con
   }
   double smin_n = __shfl_down(smin, 1), smax_n = __shfl_down(smax, 1);
 
-  // dt is needed only from here on: the CFL maximum is read after the state
-  // loads are in flight (main.jl:212, dt/dx of main.jl:225)
-  const double dtdx = (a.cfl * a.dx / lmax) / a.dx;
-
   // face i+1/2 (this lane) and face i-1/2 (from lane-1)
-  double upd[NC];
   if (FLUX == FLUX_RUSANOV) {
     double lam_c = fmax(fabs(smin), fabs(smax));       // maximum(abs.(eigvals))
     double lam_n = fmax(fabs(smin_n), fabs(smax_n));
@@ -224,6 +240,7 @@ __device__ __forceinline__ double chunk_step(const StepArgs& a, long long i, con
       double dpl = __shfl_up(dp[r], 1);
       upd[r] = q[r] - dtdx * (dm[r] + dpl);  // conservative part is zero (NumFluxes.jl:82)
     }
+  }
   }
 
   const bool owned = inr && lane >= 1 && lane <= TILE;
@@ -300,37 +317,6 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : 1))) void 
   load_chunk<NPH>(a, i, q, cc);
   const double lmax = step_lmax(a);
   double lam = chunk_step<NPH, FLUX, FAST>(a, i, q, cc, lmax);
-  step_epilogue(a, lam, tcur, lmax);
-}
-
-// A strip of consecutive chunks per wave; the next chunk's state is loaded
-// while the current chunk computes.
-template <int NPH, int FLUX, int FAST>
-__global__ __launch_bounds__(BLOCK)
-__attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : 1))) void k_step_strip(
-    StepArgs a, int chunks_per_wave) {
-  constexpr int NC = NComp<NPH>::value;
-  const int lane = threadIdx.x & 63;
-  const long long wave = (xcd_block() * BLOCK + threadIdx.x) >> 6;
-  const long long nchunks = (a.n + TILE - 1) / TILE;
-  long long c = wave * chunks_per_wave;
-  const long long cend = c + chunks_per_wave < nchunks ? c + chunks_per_wave : nchunks;
-  double tcur;
-  if (!step_begin(a, tcur)) return;
-  double lam = 0.0;
-  double q[NC], cc[NPH];
-  if (c < cend) load_chunk<NPH>(a, c * TILE + lane - 1, q, cc);
-  const double lmax = step_lmax(a);
-#pragma unroll 1
-  for (; c < cend; ++c) {
-    double qn[NC], ccn[NPH];
-    if (c + 1 < cend) load_chunk<NPH>(a, (c + 1) * TILE + lane - 1, qn, ccn);
-    lam = fmax(lam, chunk_step<NPH, FLUX, FAST>(a, c * TILE + lane - 1, q, cc, lmax));
-#pragma unroll
-    for (int r = 0; r < NC; ++r) q[r] = qn[r];
-#pragma unroll
-    for (int p = 0; p < NPH; ++p) cc[p] = ccn[p];
-  }
   step_epilogue(a, lam, tcur, lmax);
 }
 
@@ -641,16 +627,6 @@ namespace {
 template <int NPH, int FLUX>
 void launch_step_t(const StepArgs& a, long long nchunks, bool fast, hipStream_t s) {
   constexpr int WPB = BLOCK / 64;
-  const int K = HES_STRIP;
-  if (K > 0) {
-    long long nwaves = (nchunks + K - 1) / K;
-    long long nblocks = (nwaves + WPB - 1) / WPB;
-    if (fast)
-      hipLaunchKernelGGL((k_step_strip<NPH, FLUX, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a, K);
-    else
-      hipLaunchKernelGGL((k_step_strip<NPH, FLUX, 0>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a, K);
-    return;
-  }
   long long nblocks = (nchunks + WPB - 1) / WPB;
   if (fast)
     hipLaunchKernelGGL((k_step<NPH, FLUX, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
