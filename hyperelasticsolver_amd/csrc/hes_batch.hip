@@ -1,0 +1,607 @@
+// hes_batch.hip -- many independent 1D problems of uneven length advanced
+// together: one launch per time step covers every problem.
+//
+// All cells of all problems are packed into one structure-of-arrays state
+// (problem p owns cells [offsets[p], offsets[p+1])).  Work is cut into tiles of
+// 62 owned cells that never straddle two problems, so within a wave the
+// material constants, dx, CFL and dt are uniform and come from scalar loads.
+// Each problem keeps its own CFL ring (max lambda of state s in
+// slots[3p + s % 3]) and time ring, i.e. its own dt (main.jl:204-214 per
+// problem) and its own frozen end cells (main.jl:219-220).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "hes_host.hpp"
+#include "hes_step.hpp"
+#include "../../include/hes.h"
+
+using namespace hes;
+
+struct BatchArgs {
+  const double* __restrict__ Qin;
+  double* __restrict__ Qout;
+  const double* __restrict__ Cin;
+  double* __restrict__ Cout;
+  const long long* __restrict__ tile_start;  // first owned cell of each tile
+  const int* __restrict__ tile_prob;         // problem of each tile
+  const long long* __restrict__ offsets;     // P + 1
+  const MatDev* __restrict__ mats;           // nphase per problem
+  const double* __restrict__ cfl;
+  const double* __restrict__ dx;
+  const double* __restrict__ t_end;
+  unsigned long long* slots;                 // 3 per problem
+  double* tring;                             // 2 per problem
+  int* psteps;                               // executed steps per problem
+  int* status;                               // [0] error bits
+  long long pitch;
+  int tile_begin, tile_end;
+  int step;  // ring phase (step index modulo 6)
+  int xcd;   // deal consecutive blocks to one XCD
+};
+
+template <int BLK>
+__device__ __forceinline__ int batch_tile(const BatchArgs& a) {
+  long long b = a.xcd ? xcd_block() : (long long)blockIdx.x;
+  int w = (int)((b * BLK + threadIdx.x) >> 6);
+  return __builtin_amdgcn_readfirstlane(a.tile_begin + w);
+}
+
+__device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double m) {
+  if (m > 0.0) {
+    unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+    unsigned long long cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (bits > cur) atomicMax(slot, bits);
+  }
+}
+
+template <int NPH, int FLUX, int FAST, int BLK>
+__global__ __launch_bounds__(BLK)
+__attribute__((amdgpu_waves_per_eu(NPH == 1 ? 3 : 1))) void k_batch_step(BatchArgs a) {
+  constexpr int NC = NComp<NPH>::value;
+  const int lane = threadIdx.x & 63;
+  const int tile = batch_tile<BLK>(a);
+  if (tile >= a.tile_end) return;  // the whole wave (no block-level sync here)
+  const int p = a.tile_prob[tile];
+  const long long lo = a.offsets[p], hi = a.offsets[p + 1];
+  const long long first = a.tile_start[tile];
+  const bool lead = first == lo;  // this tile does the problem's bookkeeping
+  const double tcur = a.tring[2 * p + (a.step & 1)];
+  if (!(tcur < a.t_end[p])) {
+    if (lead && lane == 0) a.tring[2 * p + ((a.step + 1) & 1)] = tcur;
+    return;
+  }
+  MatDev m[NPH];
+#pragma unroll
+  for (int k = 0; k < NPH; ++k) m[k] = a.mats[NPH * p + k];
+  const ChunkCtx ctx = {m, a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, lo, hi, a.cfl[p], a.dx[p]};
+  const long long i = first + lane - 1;
+  double q[NC], cc[NPH];
+  load_chunk<NPH>(ctx, i, q, cc);
+  const double lmax = __longlong_as_double(*(const long long*)(a.slots + 3 * p + (a.step % 3)));
+  double lam = chunk_step<NPH, FLUX, FAST>(ctx, i, q, cc, lmax);
+  lam = wave_max(lam);
+  if (lane == 0) {
+    atomic_max_pos(a.slots + 3 * p + ((a.step + 1) % 3), lam);
+    if (lead) {
+      const double dt = ctx.cfl * ctx.dx / lmax;
+      a.tring[2 * p + ((a.step + 1) & 1)] = tcur + dt;
+      a.slots[3 * p + ((a.step + 2) % 3)] = 0ull;
+      a.psteps[p] += 1;
+    }
+  }
+}
+
+// Sound speeds and per-problem CFL maximum of the current state.
+template <int NPH>
+__global__ __launch_bounds__(64) void k_batch_speeds(BatchArgs a) {
+  constexpr int NC = NComp<NPH>::value;
+  const int lane = threadIdx.x & 63;
+  const int tile = __builtin_amdgcn_readfirstlane(a.tile_begin + (int)blockIdx.x);
+  if (tile >= a.tile_end) return;
+  const int p = a.tile_prob[tile];
+  const long long lo = a.offsets[p], hi = a.offsets[p + 1];
+  const long long i = a.tile_start[tile] + lane - 1;
+  MatDev m[NPH];
+#pragma unroll
+  for (int k = 0; k < NPH; ++k) m[k] = a.mats[NPH * p + k];
+  double lam = 0.0;
+  int bad = 0;
+  if (lane >= 1 && lane <= TILE && i < hi) {
+    double q[NC], cc[NPH];
+#pragma unroll
+    for (int r = 0; r < NC; ++r) q[r] = a.Qin[r * a.pitch + i];
+    lam = cell_speeds<NPH, -1>(m, q, cc, bad);
+#pragma unroll
+    for (int k = 0; k < NPH; ++k) a.Cout[k * a.pitch + i] = cc[k];
+  }
+  (void)lo;
+  if (__any(bad) && lane == 0) atomicOr(a.status, 1);
+  lam = wave_max(lam);
+  if (lane == 0) atomic_max_pos(a.slots + 3 * p + (a.step % 3), lam);
+}
+
+// Piecewise-constant initial state per problem: problem p has nstates[p]
+// states (states + nc * soff[p]) and nstates[p]-1 sorted interfaces
+// (ifc + soff[p] - p); cell j of the problem (centre (j+1/2)/n_p) takes state
+// #{k : centre > s_k}.
+__global__ void k_batch_piecewise(const long long* __restrict__ offsets, int P,
+                                  const int* __restrict__ soff, const double* __restrict__ states,
+                                  const double* __restrict__ ifc, int nc, double* __restrict__ Q,
+                                  long long N, long long pitch) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int lo = 0, hi = P;  // largest p with offsets[p] <= i
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (offsets[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  const int p = lo;
+  const long long n = offsets[p + 1] - offsets[p];
+  const double x = ((double)(i - offsets[p]) + 0.5) / (double)n;
+  const int k0 = soff[p], nk = soff[p + 1] - soff[p];
+  const double* s = ifc + (k0 - p);
+  int a = 0, b = nk - 1;  // first k with !(x > s_k)
+  while (a < b) {
+    int mid = (a + b) >> 1;
+    if (x > s[mid]) a = mid + 1;
+    else b = mid;
+  }
+  const double* src = states + (long long)(k0 + a) * nc;
+  for (int r = 0; r < nc; ++r) Q[r * pitch + i] = src[r];
+}
+
+// The end cells of every problem never change: copy them to the other buffer.
+__global__ void k_batch_copy_ends(const long long* __restrict__ offsets, int P, int nc,
+                                  const double* __restrict__ src, double* __restrict__ dst,
+                                  long long pitch) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  long long a = offsets[p], b = offsets[p + 1] - 1;
+  for (int r = 0; r < nc; ++r) {
+    dst[r * pitch + a] = src[r * pitch + a];
+    dst[r * pitch + b] = src[r * pitch + b];
+  }
+}
+
+__global__ void k_soa_to_aos_b(const double* __restrict__ in, double* __restrict__ out, int nc,
+                               long long n, long long pitch) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < nc; ++r) out[i * nc + r] = in[r * pitch + i];
+}
+__global__ void k_aos_to_soa_b(const double* __restrict__ in, double* __restrict__ out, int nc,
+                               long long n, long long pitch) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < nc; ++r) out[r * pitch + i] = in[i * nc + r];
+}
+
+struct hes_batch {
+  int nph, flux, nc, P, ntiles;
+  long long N, pitch;
+  bool fast;
+  int block, xcd;  // launch shape (autotuned)
+  std::vector<long long> offsets_h;
+  std::vector<int> tile_first_h;  // first tile of each problem (P + 1)
+  double *Q[2], *C[2];
+  long long* tile_start;
+  int* tile_prob;
+  long long* offsets;
+  MatDev* mats;
+  double *cfl, *dx, *t_end, *tring;
+  unsigned long long* slots;
+  int *psteps, *status;
+  long long step;
+  hipStream_t stream;
+  hipEvent_t ev0, ev1;
+  bool has_state;
+};
+
+namespace {
+
+BatchArgs batch_args(hes_batch* h, long long s, int tb, int te) {
+  BatchArgs a;
+  a.Qin = h->Q[s & 1];
+  a.Qout = h->Q[(s + 1) & 1];
+  a.Cin = h->C[s & 1];
+  a.Cout = h->C[(s + 1) & 1];
+  a.tile_start = h->tile_start;
+  a.tile_prob = h->tile_prob;
+  a.offsets = h->offsets;
+  a.mats = h->mats;
+  a.cfl = h->cfl;
+  a.dx = h->dx;
+  a.t_end = h->t_end;
+  a.slots = h->slots;
+  a.tring = h->tring;
+  a.psteps = h->psteps;
+  a.status = h->status;
+  a.pitch = h->pitch;
+  a.tile_begin = tb;
+  a.tile_end = te;
+  a.step = (int)(s % 6);
+  a.xcd = h->xcd;
+  return a;
+}
+
+template <int NPH, int FLUX, int FAST>
+void launch_batch_t(const BatchArgs& a, int block, hipStream_t st) {
+  int nt = a.tile_end - a.tile_begin;
+  if (nt <= 0) return;
+  if (block == 64)
+    hipLaunchKernelGGL((k_batch_step<NPH, FLUX, FAST, 64>), dim3((unsigned)nt), dim3(64), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_batch_step<NPH, FLUX, FAST, 256>), dim3((unsigned)((nt + 3) / 4)),
+                       dim3(256), 0, st, a);
+}
+
+void launch_batch(hes_batch* h, const BatchArgs& a) {
+  if (h->nph == 1) {
+    if (h->flux == HES_FLUX_RUSANOV)
+      h->fast ? launch_batch_t<1, FLUX_RUSANOV, 1>(a, h->block, h->stream)
+              : launch_batch_t<1, FLUX_RUSANOV, 0>(a, h->block, h->stream);
+    else
+      h->fast ? launch_batch_t<1, FLUX_HLL, 1>(a, h->block, h->stream)
+              : launch_batch_t<1, FLUX_HLL, 0>(a, h->block, h->stream);
+  } else {
+    if (h->flux == HES_FLUX_RUSANOV)
+      h->fast ? launch_batch_t<2, FLUX_RUSANOV, 1>(a, h->block, h->stream)
+              : launch_batch_t<2, FLUX_RUSANOV, 0>(a, h->block, h->stream);
+    else
+      h->fast ? launch_batch_t<2, FLUX_HLL, 1>(a, h->block, h->stream)
+              : launch_batch_t<2, FLUX_HLL, 0>(a, h->block, h->stream);
+  }
+}
+
+int batch_init_speeds(hes_batch* h) {
+  long long s = h->step;
+  HIP_TRY(hipMemsetAsync(h->slots, 0, sizeof(unsigned long long) * 3 * h->P, h->stream));
+  BatchArgs a = batch_args(h, s, 0, h->ntiles);
+  a.Cout = h->C[s & 1];
+  if (h->nph == 1)
+    hipLaunchKernelGGL(k_batch_speeds<1>, dim3((unsigned)h->ntiles), dim3(64), 0, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_batch_speeds<2>, dim3((unsigned)h->ntiles), dim3(64), 0, h->stream, a);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_batch_copy_ends, dim3((unsigned)((h->P + 127) / 128)), dim3(128), 0, h->stream,
+                     h->offsets, h->P, h->nc, h->Q[s & 1], h->Q[(s + 1) & 1], h->pitch);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  int st = 0;
+  HIP_TRY(hipMemcpy(&st, h->status, sizeof(int), hipMemcpyDeviceToHost));
+  if (st) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
+  return HES_OK;
+}
+
+int batch_after_set(hes_batch* h, const double* t0) {
+  std::vector<double> tr(2 * h->P);
+  for (int p = 0; p < h->P; ++p) tr[2 * p] = tr[2 * p + 1] = t0 ? t0[p] : 0.0;
+  HIP_TRY(hipMemcpy(h->tring, tr.data(), sizeof(double) * 2 * h->P, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(h->psteps, 0, sizeof(int) * h->P));
+  HIP_TRY(hipMemset(h->status, 0, sizeof(int) * 4));
+  h->step = 0;
+  h->has_state = true;
+  return batch_init_speeds(h);
+}
+
+}  // namespace
+
+extern "C" {
+
+int hes_batch_destroy(hes_batch* h) {
+  if (!h) return HES_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  void* ptrs[] = {h->Q[0], h->Q[1], h->C[0], h->C[1], h->tile_start, h->tile_prob, h->offsets,
+                  h->mats, h->cfl, h->dx, h->t_end, h->tring, h->slots, h->psteps, h->status};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return HES_OK;
+}
+
+int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const long long* ncells,
+                     const double* materials, const double* cfl, const double* dx) {
+  if (!out) return fail(HES_ERR_ARG, "null handle pointer");
+  *out = nullptr;
+  if (nphase != 1 && nphase != 2) return fail(HES_ERR_ARG, "nphase must be 1 or 2");
+  if (flux != HES_FLUX_RUSANOV && flux != HES_FLUX_HLL) return fail(HES_ERR_ARG, "unknown flux");
+  if (nproblems < 1 || !ncells || !materials) return fail(HES_ERR_ARG, "null argument");
+  hes_batch* h = new hes_batch();
+  h->nph = nphase;
+  h->flux = flux;
+  h->nc = nphase == 1 ? 14 : 30;
+  h->P = nproblems;
+  h->block = 256;
+  h->xcd = 1;
+  h->offsets_h.resize(nproblems + 1);
+  h->tile_first_h.resize(nproblems + 1);
+  h->offsets_h[0] = 0;
+  std::vector<long long> tstart;
+  std::vector<int> tprob;
+  std::vector<MatDev> mats((size_t)nproblems * nphase);
+  std::vector<double> vcfl(nproblems), vdx(nproblems), vte(nproblems, INFINITY);
+  h->fast = true;
+  for (int p = 0; p < nproblems; ++p) {
+    if (ncells[p] < 3) { delete h; return fail(HES_ERR_ARG, "every problem needs at least 3 cells"); }
+    for (int k = 0; k < nphase; ++k) {
+      const double* mp = materials + 8 * ((size_t)nphase * p + k);
+      if (!mat_ok(mp)) { delete h; return fail(HES_ERR_ARG, "invalid material constants"); }
+      mats[(size_t)nphase * p + k] = make_mat(mp);
+      h->fast = h->fast && mats[(size_t)nphase * p + k].fast;
+    }
+    vcfl[p] = cfl ? cfl[p] : 0.6;
+    vdx[p] = dx ? dx[p] : 1.0 / (double)ncells[p];
+    if (!(vcfl[p] > 0) || !(vdx[p] > 0)) { delete h; return fail(HES_ERR_ARG, "cfl and dx must be positive"); }
+    h->offsets_h[p + 1] = h->offsets_h[p] + ncells[p];
+    h->tile_first_h[p] = (int)tstart.size();
+    for (long long c = h->offsets_h[p]; c < h->offsets_h[p + 1]; c += TILE) {
+      tstart.push_back(c);
+      tprob.push_back(p);
+    }
+  }
+  h->tile_first_h[nproblems] = (int)tstart.size();
+  h->ntiles = (int)tstart.size();
+  h->N = h->offsets_h[nproblems];
+  h->pitch = (h->N + 63) / 64 * 64;
+  size_t qb = sizeof(double) * (size_t)h->nc * h->pitch, cb = sizeof(double) * (size_t)nphase * h->pitch;
+  hipError_t e = hipSuccess;
+  for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+    e = hipMalloc(&h->Q[b], qb);
+    if (e == hipSuccess) e = hipMalloc(&h->C[b], cb);
+  }
+  auto up = [&](void** d, const void* src, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(d, bytes);
+    if (e == hipSuccess && src) e = hipMemcpy(*d, src, bytes, hipMemcpyHostToDevice);
+  };
+  up((void**)&h->tile_start, tstart.data(), sizeof(long long) * tstart.size());
+  up((void**)&h->tile_prob, tprob.data(), sizeof(int) * tprob.size());
+  up((void**)&h->offsets, h->offsets_h.data(), sizeof(long long) * (nproblems + 1));
+  up((void**)&h->mats, mats.data(), sizeof(MatDev) * mats.size());
+  up((void**)&h->cfl, vcfl.data(), sizeof(double) * nproblems);
+  up((void**)&h->dx, vdx.data(), sizeof(double) * nproblems);
+  up((void**)&h->t_end, vte.data(), sizeof(double) * nproblems);
+  up((void**)&h->tring, nullptr, sizeof(double) * 2 * nproblems);
+  up((void**)&h->slots, nullptr, sizeof(unsigned long long) * 3 * nproblems);
+  up((void**)&h->psteps, nullptr, sizeof(int) * nproblems);
+  up((void**)&h->status, nullptr, sizeof(int) * 4);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e != hipSuccess) {
+    g_err = std::string("batch allocation failed: ") + hipGetErrorString(e);
+    hes_batch_destroy(h);
+    return HES_ERR_HIP;
+  }
+  *out = h;
+  return HES_OK;
+}
+
+int hes_batch_info(hes_batch* h, long long* total_cells, int* ntiles, int* block, int* xcd) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  if (total_cells) *total_cells = h->N;
+  if (ntiles) *ntiles = h->ntiles;
+  if (block) *block = h->block;
+  if (xcd) *xcd = h->xcd;
+  return HES_OK;
+}
+
+int hes_batch_set_t_end(hes_batch* h, const double* t_end) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  std::vector<double> v(h->P, INFINITY);
+  if (t_end)
+    for (int p = 0; p < h->P; ++p) v[p] = t_end[p];
+  HIP_TRY(hipMemcpy(h->t_end, v.data(), sizeof(double) * h->P, hipMemcpyHostToDevice));
+  return HES_OK;
+}
+
+int hes_batch_set_state(hes_batch* h, const double* Q, const double* t0) {
+  if (!h || !Q) return fail(HES_ERR_ARG, "null argument");
+  size_t bytes = sizeof(double) * (size_t)h->nc * h->N;
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, bytes));
+  hipError_t e = hipMemcpy(tmp, Q, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_aos_to_soa_b, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->stream,
+                       tmp, h->Q[0], h->nc, h->N, h->pitch);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) {
+    g_err = std::string("batch set_state: ") + hipGetErrorString(e);
+    return HES_ERR_HIP;
+  }
+  return batch_after_set(h, t0);
+}
+
+int hes_batch_set_piecewise(hes_batch* h, const int* nstates, const double* states,
+                            const double* interfaces, const double* t0) {
+  if (!h || !nstates || !states) return fail(HES_ERR_ARG, "null argument");
+  std::vector<int> soff(h->P + 1, 0);
+  for (int p = 0; p < h->P; ++p) {
+    if (nstates[p] < 1) return fail(HES_ERR_ARG, "every problem needs at least one state");
+    soff[p + 1] = soff[p] + nstates[p];
+  }
+  int S = soff[h->P], I = S - h->P;
+  if (I > 0 && !interfaces) return fail(HES_ERR_ARG, "null interfaces");
+  for (int p = 0; p < h->P; ++p)
+    for (int k = soff[p] - p; k + 1 < soff[p + 1] - p - 1; ++k)
+      if (!(interfaces[k] <= interfaces[k + 1])) return fail(HES_ERR_ARG, "interfaces not sorted");
+  int *dso = nullptr;
+  double *dst = nullptr, *dif = nullptr;
+  HIP_TRY(hipMalloc(&dso, sizeof(int) * (h->P + 1)));
+  hipError_t e = hipMalloc(&dst, sizeof(double) * (size_t)S * h->nc);
+  if (e == hipSuccess) e = hipMalloc(&dif, sizeof(double) * (size_t)(I > 0 ? I : 1));
+  if (e == hipSuccess) e = hipMemcpy(dso, soff.data(), sizeof(int) * (h->P + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dst, states, sizeof(double) * (size_t)S * h->nc, hipMemcpyHostToDevice);
+  if (e == hipSuccess && I > 0) e = hipMemcpy(dif, interfaces, sizeof(double) * I, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_batch_piecewise, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0,
+                       h->stream, h->offsets, h->P, dso, dst, dif, h->nc, h->Q[0], h->N, h->pitch);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(dso);
+  (void)hipFree(dst);
+  (void)hipFree(dif);
+  if (e != hipSuccess) {
+    g_err = std::string("batch set_piecewise: ") + hipGetErrorString(e);
+    return HES_ERR_HIP;
+  }
+  return batch_after_set(h, t0);
+}
+
+int hes_batch_get_state(hes_batch* h, double* Q) {
+  if (!h || !Q) return fail(HES_ERR_ARG, "null argument");
+  if (!h->has_state) return fail(HES_ERR_ARG, "no state set");
+  size_t bytes = sizeof(double) * (size_t)h->nc * h->N;
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, bytes));
+  hipLaunchKernelGGL(k_soa_to_aos_b, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->stream,
+                     h->Q[h->step & 1], tmp, h->nc, h->N, h->pitch);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(Q, tmp, bytes, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) {
+    g_err = std::string("batch get_state: ") + hipGetErrorString(e);
+    return HES_ERR_HIP;
+  }
+  return HES_OK;
+}
+
+int hes_batch_get_times(hes_batch* h, double* t, long long* steps) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  std::vector<double> tr(2 * h->P);
+  std::vector<int> ps(h->P);
+  HIP_TRY(hipMemcpy(tr.data(), h->tring, sizeof(double) * 2 * h->P, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ps.data(), h->psteps, sizeof(int) * h->P, hipMemcpyDeviceToHost));
+  for (int p = 0; p < h->P; ++p) {
+    if (t) t[p] = tr[2 * p + (h->step & 1)];
+    if (steps) steps[p] = ps[p];
+  }
+  return HES_OK;
+}
+
+int hes_batch_set_launch(hes_batch* h, int block, int xcd) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  if (block != 64 && block != 256) return fail(HES_ERR_ARG, "block must be 64 or 256");
+  h->block = block;
+  h->xcd = xcd ? 1 : 0;
+  return HES_OK;
+}
+
+// mode 0: one batched launch per step; 1: the whole step loop captured in one
+// hipGraph and launched once; 2: one launch per problem per step.
+int hes_batch_run(hes_batch* h, long long steps, int mode, float* elapsed_ms) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  if (!h->has_state) return fail(HES_ERR_ARG, "no state set");
+  if (steps < 0 || mode < 0 || mode > 2) return fail(HES_ERR_ARG, "bad argument");
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  if (mode == 1 && steps > 0) {
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (long long j = 0; j < steps; ++j)
+      launch_batch(h, batch_args(h, h->step + j, 0, h->ntiles));
+    hipError_t e = hipStreamEndCapture(h->stream, &graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      g_err = std::string("batch graph: ") + hipGetErrorString(e);
+      return HES_ERR_HIP;
+    }
+  }
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  hipError_t e = hipSuccess;
+  if (mode == 1 && steps > 0) {
+    e = hipGraphLaunch(gexec, h->stream);
+  } else {
+    for (long long j = 0; j < steps && e == hipSuccess; ++j) {
+      if (mode == 0) {
+        launch_batch(h, batch_args(h, h->step + j, 0, h->ntiles));
+      } else {
+        for (int p = 0; p < h->P; ++p)
+          launch_batch(h, batch_args(h, h->step + j, h->tile_first_h[p], h->tile_first_h[p + 1]));
+      }
+      e = hipGetLastError();
+    }
+  }
+  if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    g_err = std::string("batch run: ") + hipGetErrorString(e);
+    return HES_ERR_HIP;
+  }
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (elapsed_ms) *elapsed_ms = ms;
+  h->step += steps;
+  int st = 0;
+  HIP_TRY(hipMemcpy(&st, h->status, sizeof(int), hipMemcpyDeviceToHost));
+  if (st) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
+  return HES_OK;
+}
+
+// Try every launch shape for `steps` steps on a copy of the current state and
+// keep the fastest.  The state, times and CFL rings are restored afterwards.
+int hes_batch_autotune(hes_batch* h, int steps, int* best_block, int* best_xcd, float* best_ms) {
+  if (!h || steps < 1) return fail(HES_ERR_ARG, "bad argument");
+  if (!h->has_state) return fail(HES_ERR_ARG, "no state set");
+  long long s0 = h->step;
+  size_t qb = sizeof(double) * (size_t)h->nc * h->pitch, cb = sizeof(double) * (size_t)h->nph * h->pitch;
+  double *q0 = nullptr, *c0 = nullptr, *t0 = nullptr;
+  unsigned long long* sl0 = nullptr;
+  int* ps0 = nullptr;
+  HIP_TRY(hipMalloc(&q0, qb));
+  HIP_TRY(hipMalloc(&c0, cb));
+  HIP_TRY(hipMalloc(&t0, sizeof(double) * 2 * h->P));
+  HIP_TRY(hipMalloc(&sl0, sizeof(unsigned long long) * 3 * h->P));
+  HIP_TRY(hipMalloc(&ps0, sizeof(int) * h->P));
+  HIP_TRY(hipMemcpy(q0, h->Q[s0 & 1], qb, hipMemcpyDeviceToDevice));
+  HIP_TRY(hipMemcpy(c0, h->C[s0 & 1], cb, hipMemcpyDeviceToDevice));
+  HIP_TRY(hipMemcpy(t0, h->tring, sizeof(double) * 2 * h->P, hipMemcpyDeviceToDevice));
+  HIP_TRY(hipMemcpy(sl0, h->slots, sizeof(unsigned long long) * 3 * h->P, hipMemcpyDeviceToDevice));
+  HIP_TRY(hipMemcpy(ps0, h->psteps, sizeof(int) * h->P, hipMemcpyDeviceToDevice));
+  const int blocks[2] = {64, 256};
+  float best = 1e30f;
+  int bb = h->block, bx = h->xcd, rc = HES_OK;
+  for (int bi = 0; bi < 2 && rc == HES_OK; ++bi)
+    for (int x = 0; x < 2 && rc == HES_OK; ++x) {
+      h->block = blocks[bi];
+      h->xcd = x;
+      float ms = 0.f;
+      rc = hes_batch_run(h, 1, 0, &ms);  // warm-up
+      if (rc == HES_OK) rc = hes_batch_run(h, steps, 0, &ms);
+      if (rc == HES_OK && ms < best) { best = ms; bb = blocks[bi]; bx = x; }
+      // restore the starting state for the next candidate
+      h->step = s0;
+      if (hipMemcpy(h->Q[s0 & 1], q0, qb, hipMemcpyDeviceToDevice) != hipSuccess ||
+          hipMemcpy(h->C[s0 & 1], c0, cb, hipMemcpyDeviceToDevice) != hipSuccess ||
+          hipMemcpy(h->tring, t0, sizeof(double) * 2 * h->P, hipMemcpyDeviceToDevice) != hipSuccess ||
+          hipMemcpy(h->slots, sl0, sizeof(unsigned long long) * 3 * h->P, hipMemcpyDeviceToDevice) != hipSuccess ||
+          hipMemcpy(h->psteps, ps0, sizeof(int) * h->P, hipMemcpyDeviceToDevice) != hipSuccess)
+        rc = fail(HES_ERR_HIP, "autotune restore failed");
+    }
+  (void)hipFree(q0);
+  (void)hipFree(c0);
+  (void)hipFree(t0);
+  (void)hipFree(sl0);
+  (void)hipFree(ps0);
+  h->block = bb;
+  h->xcd = bx;
+  if (best_block) *best_block = bb;
+  if (best_xcd) *best_xcd = bx;
+  if (best_ms) *best_ms = best / (float)steps;
+  return rc;
+}
+
+}  // extern "C"

@@ -21,15 +21,15 @@ namespace hes {
 #define HES_LO_W1 0.37847495629784698032
 #define HES_LO_W2 0.55485837703548635301
 
-__constant__ double c_gl_s[6] = {(-HES_GL_X0 + 1.0) / 2.0, (-HES_GL_X1 + 1.0) / 2.0,
+static __constant__ double c_gl_s[6] = {(-HES_GL_X0 + 1.0) / 2.0, (-HES_GL_X1 + 1.0) / 2.0,
                                  (-HES_GL_X2 + 1.0) / 2.0, (HES_GL_X2 + 1.0) / 2.0,
                                  (HES_GL_X1 + 1.0) / 2.0,  (HES_GL_X0 + 1.0) / 2.0};
-__constant__ double c_gl_w[6] = {HES_GL_W0 / 2.0, HES_GL_W1 / 2.0, HES_GL_W2 / 2.0,
+static __constant__ double c_gl_w[6] = {HES_GL_W0 / 2.0, HES_GL_W1 / 2.0, HES_GL_W2 / 2.0,
                                  HES_GL_W2 / 2.0, HES_GL_W1 / 2.0, HES_GL_W0 / 2.0};
-__constant__ double c_lo_s[6] = {(-1.0 + 1.0) / 2.0,        (-HES_LO_X1 + 1.0) / 2.0,
+static __constant__ double c_lo_s[6] = {(-1.0 + 1.0) / 2.0,        (-HES_LO_X1 + 1.0) / 2.0,
                                  (-HES_LO_X2 + 1.0) / 2.0, (HES_LO_X2 + 1.0) / 2.0,
                                  (HES_LO_X1 + 1.0) / 2.0,  (1.0 + 1.0) / 2.0};
-__constant__ double c_lo_w[6] = {HES_LO_W0 / 2.0, HES_LO_W1 / 2.0, HES_LO_W2 / 2.0,
+static __constant__ double c_lo_w[6] = {HES_LO_W0 / 2.0, HES_LO_W1 / 2.0, HES_LO_W2 / 2.0,
                                  HES_LO_W2 / 2.0, HES_LO_W1 / 2.0, HES_LO_W0 / 2.0};
 
 enum { FLUX_RUSANOV = 0, FLUX_HLL = 1 };

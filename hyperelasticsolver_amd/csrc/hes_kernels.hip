@@ -22,68 +22,19 @@
 
 #include "hes_device.hpp"
 #include "hes_faces.hpp"
+#include "hes_host.hpp"
+#include "hes_step.hpp"
 #include "../../include/hes.h"
 
 using namespace hes;
 
-namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const char* msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIP_TRY(expr)                                                              \
-  do {                                                                             \
-    hipError_t e__ = (expr);                                                       \
-    if (e__ != hipSuccess) {                                                       \
-      g_err = std::string(#expr) + ": " + hipGetErrorString(e__);                  \
-      return HES_ERR_HIP;                                                          \
-    }                                                                              \
-  } while (0)
-
-constexpr int TILE = 62;      // owned cells per wave
 #ifndef HES_STEP_BLOCK
 #define HES_STEP_BLOCK 256
 #endif
 constexpr int BLOCK = HES_STEP_BLOCK;  // threads per block of the step kernel
-
-MatDev make_mat(const double* p) {
-  MatDev m;
-  std::memset(&m, 0, sizeof(m));
-  m.rho0 = p[0];
-  m.irho0 = 1.0 / p[0];
-  m.cv = p[2];
-  m.t0 = p[3];
-  double b0 = p[4];
-  m.alpha = p[5];
-  m.beta = p[6];
-  m.gamma = p[7];
-  m.b0sq = b0 * b0;  // EquationsOfState.jl:111-112
-  m.k0 = p[1] * p[1] - (4.0 / 3.0) * (b0 * b0);
-  m.hk0a2 = 0.5 * m.k0 / (m.alpha * m.alpha);
-  m.k0_2a = m.k0 / (2.0 * m.alpha);
-  m.k0_2 = 0.5 * m.k0;
-  m.cvt0 = m.cv * m.t0;
-  m.icvt0 = 1.0 / m.cvt0;
-  m.hb0sq = 0.5 * m.b0sq;
-  m.half_a = 0.5 * m.alpha;
-  m.half_b = 0.5 * m.beta;
-  m.half_g = 0.5 * m.gamma;
-  m.g2_2 = 0.5 * m.gamma * m.gamma;
-  m.fast = (m.alpha == 1.0 && m.beta == 3.0 && m.gamma == 2.0) ? 1 : 0;
-  return m;
-}
-
-bool mat_ok(const double* p) {
-  for (int k = 0; k < 8; ++k)
-    if (!std::isfinite(p[k])) return false;
-  return p[0] > 0 && p[2] > 0 && p[3] > 0 && p[5] != 0;
-}
-
-}  // namespace
+#ifndef HES_STEP_WAVES_PER_EU
+#define HES_STEP_WAVES_PER_EU 3
+#endif
 
 // ---------------------------------------------------------------------------
 // Step kernel
@@ -104,11 +55,6 @@ struct StepArgs {
   MatDev mat[2];
 };
 
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-}
 
 __device__ __forceinline__ void block_max_to_slot(double v, unsigned long long* slot) {
   double m;
@@ -131,147 +77,6 @@ __device__ __forceinline__ void block_max_to_slot(double v, unsigned long long* 
       if (bits > cur) atomicMax(slot, bits);
     }
   }
-}
-
-// Streaming access to the state: each element is read once and written once
-// per step, so it bypasses cache allocation.
-#ifndef HES_NT_LOAD
-#define HES_NT_LOAD 1
-#endif
-#ifndef HES_NT_STORE
-#define HES_NT_STORE 0
-#endif
-__device__ __forceinline__ double ld_stream(const double* p) {
-#if HES_NT_LOAD
-  return __builtin_nontemporal_load(p);
-#else
-  return *p;
-#endif
-}
-__device__ __forceinline__ void st_stream(double v, double* p) {
-#if HES_NT_STORE
-  __builtin_nontemporal_store(v, p);
-#else
-  *p = v;
-#endif
-}
-
-// Block order across the 8 XCDs: the hardware deals consecutive blocks to
-// XCDs round-robin; remap so each XCD walks a contiguous stretch of cells and
-// the cache lines two neighbouring tiles share stay in one L2.
-__device__ __forceinline__ long long xcd_block() {
-  const unsigned nb = gridDim.x, b = blockIdx.x;
-  const unsigned q = nb / 8;
-  if (b < 8 * q) return (long long)(b % 8) * q + b / 8;
-  return b;
-}
-
-#ifndef HES_STEP_WAVES_PER_EU
-#define HES_STEP_WAVES_PER_EU 3
-#endif
-
-// One 64-lane chunk: lane l holds cell i = first + l - 1 (lanes 0 and 63 are
-// halo cells); q and cc are its state and sound speeds.  Updates the owned
-// interior cells, writes their new state and sound speeds, and returns the
-// lane's CFL speed of the new state (0 for lanes that own nothing).
-template <int NPH, int FLUX, int FAST>
-__device__ __forceinline__ double chunk_step(const StepArgs& a, long long i, const double* q,
-                                             const double* cc, double lmax) {
-  constexpr int NC = NComp<NPH>::value;
-  const int lane = threadIdx.x & 63;
-  const long long n = a.n, pitch = a.pitch;
-  const bool inr = (i >= 0) && (i < n);
-
-  double f[NC], smin, smax;
-  int bad;
-  cell_flux<NPH, FAST>(a.mat, q, cc, f, smin, smax, bad);
-
-  // dt is needed only from here on: the CFL maximum is read after the state
-  // loads are in flight (main.jl:212, dt/dx of main.jl:225)
-  const double dtdx = (a.cfl * a.dx / lmax) / a.dx;
-
-  double upd[NC];
-  if (FLUX == FLUX_RUSANOV && NPH == 1) {
-    // Rusanov face i+1/2: 1/2 (F_i + F_i+1) - 1/2 lam (Q_i+1 - Q_i) with
-    // lam = max(lam_i, lam_i+1) (NumFluxes.jl:30, main.jl:47-48), split as
-    // a_i + b_i+1 with a = F/2 + lam/2 Q and b = F/2 - lam/2 Q.  Each face sum
-    // is formed from the same two rounded halves by both of its cells, so the
-    // two cells see bit-identical face fluxes.
-    double lam_c = fmax(fabs(smin), fabs(smax));  // maximum(abs.(eigvals))
-    double hr = 0.5 * fmax(lam_c, __shfl_down(lam_c, 1));
-    double hl = 0.5 * fmax(__shfl_up(lam_c, 1), lam_c);
-#pragma unroll
-    for (int r = 0; r < NC; ++r) {
-      double fa = 0.5 * f[r] + hr * q[r];
-      double fb = 0.5 * f[r] - hl * q[r];
-      double Fr = fa + __shfl_down(fb, 1);
-      double Fl = __shfl_up(fa, 1) + fb;
-      upd[r] = q[r] - dtdx * (Fr - Fl);  // main.jl:59 with zero path integrals
-    }
-  } else {
-
-  // right neighbour (cell i+1) from lane+1
-  double qn[NC], fn[NC];
-#pragma unroll
-  for (int r = 0; r < NC; ++r) {
-    qn[r] = __shfl_down(q[r], 1);
-    fn[r] = __shfl_down(f[r], 1);
-  }
-  double smin_n = __shfl_down(smin, 1), smax_n = __shfl_down(smax, 1);
-
-  // face i+1/2 (this lane) and face i-1/2 (from lane-1)
-  if (FLUX == FLUX_RUSANOV) {
-    double lam_c = fmax(fabs(smin), fabs(smax));       // maximum(abs.(eigvals))
-    double lam_n = fmax(fabs(smin_n), fabs(smax_n));
-    double cons[NC], dmp[NC];
-    face_rusanov<NPH, FAST>(a.mat, q, qn, f, fn, fmax(lam_c, lam_n), cons, dmp);
-#pragma unroll
-    for (int r = 0; r < NC; ++r) {
-      double cl = __shfl_up(cons[r], 1);
-      double nfr = NPH == 2 ? dmp[r] : 0.0;
-      double nfl = NPH == 2 ? __shfl_up(dmp[r], 1) : 0.0;
-      upd[r] = q[r] - dtdx * ((cons[r] - cl) + (nfr + nfl));  // main.jl:59
-    }
-  } else {
-    double dm[NC], dp[NC];
-    face_hll<NPH, FAST>(a.mat, q, qn, f, fn, smin, smax_n, dm, dp);
-#pragma unroll
-    for (int r = 0; r < NC; ++r) {
-      double dpl = __shfl_up(dp[r], 1);
-      upd[r] = q[r] - dtdx * (dm[r] + dpl);  // conservative part is zero (NumFluxes.jl:82)
-    }
-  }
-  }
-
-  const bool owned = inr && lane >= 1 && lane <= TILE;
-  const bool interior = owned && i >= 1 && i <= n - 2;
-  double lam_new = 0.0;
-  int bad2 = 0;
-  if (interior) {
-#pragma unroll
-    for (int r = 0; r < NC; ++r) st_stream(upd[r], a.Qout + r * pitch + i);
-    double cn[NPH];
-    lam_new = cell_speeds<NPH, FAST>(a.mat, upd, cn, bad2);
-#pragma unroll
-    for (int p = 0; p < NPH; ++p) st_stream(cn[p], a.Cout + p * pitch + i);
-  } else if (owned) {  // frozen end cells (main.jl:219-220)
-    lam_new = fmax(fabs(smin), fabs(smax));
-#pragma unroll
-    for (int p = 0; p < NPH; ++p) st_stream(cc[p], a.Cout + p * pitch + i);
-  }
-  int anybad = (owned && bad) || bad2;
-  if (__any(anybad) && lane == 0) atomicOr(a.status, 1);
-  return lam_new;
-}
-
-template <int NPH>
-__device__ __forceinline__ void load_chunk(const StepArgs& a, long long i, double* q, double* cc) {
-  constexpr int NC = NComp<NPH>::value;
-  const long long ic = i < 0 ? 0 : (i >= a.n ? a.n - 1 : i);
-#pragma unroll
-  for (int r = 0; r < NC; ++r) q[r] = ld_stream(a.Qin + r * a.pitch + ic);
-#pragma unroll
-  for (int p = 0; p < NPH; ++p) cc[p] = ld_stream(a.Cin + p * a.pitch + ic);
 }
 
 // Time bookkeeping of a step: returns false if t >= t_end (no step).
@@ -313,10 +118,11 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : 1))) void 
   const long long i = wave * TILE + lane - 1;
   double tcur;
   if (!step_begin(a, tcur)) return;
+  const ChunkCtx ctx = {a.mat, a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, 0, a.n, a.cfl, a.dx};
   double q[NC], cc[NPH];
-  load_chunk<NPH>(a, i, q, cc);
+  load_chunk<NPH>(ctx, i, q, cc);
   const double lmax = step_lmax(a);
-  double lam = chunk_step<NPH, FLUX, FAST>(a, i, q, cc, lmax);
+  double lam = chunk_step<NPH, FLUX, FAST>(ctx, i, q, cc, lmax);
   step_epilogue(a, lam, tcur, lmax);
 }
 

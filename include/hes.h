@@ -89,6 +89,42 @@ int hes_solver_time_steps(hes_solver* h, long long reps, float* elapsed_ms);
  * denominator beside the 8 TB/s specification.  No counterpart in the reference. */
 int hes_copy_bandwidth(long long bytes, int reps, double* gbs);
 
+/* ---- Batched solver: many independent 1D problems of uneven length (the
+ *      reference runs one problem per process, main.jl:130-152) advanced with
+ *      one launch per step.  Problem p has ncells[p] >= 3 cells, its own
+ *      materials (nphase blocks of 8), CFL number, dx and hence its own dt
+ *      (main.jl:204-214 per problem) and frozen end cells.  The packed state
+ *      is ncomp x sum(ncells), problems one after another (Julia layout). */
+typedef struct hes_batch hes_batch;
+
+/* cfl, dx: per problem, or NULL for 0.6 and 1/ncells[p] (main.jl:144-147). */
+int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const long long* ncells,
+                     const double* materials, const double* cfl, const double* dx);
+int hes_batch_destroy(hes_batch* h);
+int hes_batch_info(hes_batch* h, long long* total_cells, int* ntiles, int* block, int* xcd);
+/* t0: per problem start times, or NULL for 0. */
+int hes_batch_set_state(hes_batch* h, const double* Q_host, const double* t0);
+/* Per-problem piecewise-constant state built on the device: problem p has
+ * nstates[p] conservative states (consecutive in `states`, ncomp each) and
+ * nstates[p]-1 sorted interfaces in (0,1) (consecutive in `interfaces`). */
+int hes_batch_set_piecewise(hes_batch* h, const int* nstates, const double* states,
+                            const double* interfaces, const double* t0);
+int hes_batch_get_state(hes_batch* h, double* Q_host);
+/* Time and executed steps of every problem. */
+int hes_batch_get_times(hes_batch* h, double* t, long long* steps);
+/* Per-problem end time (main.jl:202 `while t < T`), NULL for none. */
+int hes_batch_set_t_end(hes_batch* h, const double* t_end);
+/* Launch shape: block of 64 or 256 threads; xcd != 0 deals consecutive blocks
+ * to one XCD. */
+int hes_batch_set_launch(hes_batch* h, int block, int xcd);
+/* Advance every problem by `steps` steps.  mode 0: one launch per step;
+ * 1: the whole step loop captured in one hipGraph and launched once;
+ * 2: one launch per problem per step (the per-problem loop). */
+int hes_batch_run(hes_batch* h, long long steps, int mode, float* elapsed_ms);
+/* Time every launch shape for `steps` steps on the current state (which is
+ * restored afterwards) and keep the fastest. */
+int hes_batch_autotune(hes_batch* h, int steps, int* block, int* xcd, float* ms_per_step);
+
 /* ---- Point-wise functions over n states (host arrays, synchronous). ---- */
 
 /* HyperelasticityMPh.cons2prim_mph (HyperelasticityMPh.jl:99-133):

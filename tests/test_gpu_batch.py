@@ -1,0 +1,148 @@
+"""Batched solver on the GPU: every problem of a batch must evolve exactly as
+it does alone (same kernels, same tiling), and match the oracle to 1e-10."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle_bind import rel_err_per_component
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _eos_list(rng, P, nph=1):
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    out = []
+    for _ in range(P):
+        s = rng.uniform(0.8, 1.2, 3)
+        e = Barton2009(_rho0=8.93 * s[0], _c0=4.6 * s[1], _b0=2.1 * s[2])
+        if nph == 1:
+            out.append(e)
+        else:
+            s2 = rng.uniform(0.8, 1.2, 3)
+            out.append((e, Barton2009(_rho0=2.7 * s2[0], _c0=4.6 * s2[1], _b0=3.1 * s2[2])))
+    return out
+
+
+def _problem_states(ora, rng, eos, nph):
+    from hyperelasticsolver_amd.solver import materials_array
+    k = int(rng.integers(1, 5))
+    mats = materials_array(eos)
+    P = np.zeros((14 if nph == 1 else 30, k), order="F")
+    for j in range(k):
+        for ph in range(nph):
+            A = np.eye(3) + 0.05 * rng.standard_normal((3, 3))
+            m = mats[8 * ph:8 * ph + 8]
+            blk = [m[0] / np.linalg.det(A), *rng.uniform(-0.5, 0.5, 3), 1e-3, *A.reshape(-1, order="F")]
+            if nph == 1:
+                P[:, j] = blk
+            else:
+                a = rng.uniform(0.01, 0.99) if ph == 0 else 1.0 - P[0, j]
+                P[15 * ph:15 * ph + 15, j] = [a] + blk
+    Qs = ora.prim2cons(nph, mats, P)
+    ifc = np.sort(rng.uniform(0.05, 0.95, k - 1))
+    return Qs, ifc
+
+
+def _build(ora, ncells, nph=1, flux="rusanov", seed=0):
+    from hyperelasticsolver_amd.batched import Batch
+    rng = np.random.default_rng(seed)
+    eos = _eos_list(rng, len(ncells), nph)
+    sts, ifs = [], []
+    for e in eos:
+        Qs, ifc = _problem_states(ora, rng, e, nph)
+        sts.append(Qs)
+        ifs.append(ifc)
+    b = Batch(eos, ncells, flux=flux)
+    b.set_piecewise(sts, ifs)
+    return b, eos, sts, ifs
+
+
+NCELLS = [3, 61, 62, 63, 64, 124, 125, 200, 517]
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "hll"])
+def test_batch_equals_single_problem_runs(ora, flux):
+    from hyperelasticsolver_amd.solver import Solver
+    print(f"[batch] vs single runs, flux={flux}", flush=True)
+    steps = 15
+    b, eos, sts, ifs = _build(ora, NCELLS, flux=flux, seed=1)
+    Q0 = b.get_state()
+    b.run(steps, mode="graph")
+    Qb = b.get_state()
+    tb, sb = b.times()
+    b.close()
+    for p, n in enumerate(NCELLS):
+        with Solver(eos[p], n, flux=flux, cfl=0.6) as s:
+            s.set_state(Q0[:, b.offsets[p]:b.offsets[p + 1]])
+            r = s.run(steps=steps, use_graph=False)
+            Qs = s.get_state()
+        assert sb[p] == steps == r["steps"]
+        assert tb[p] == r["t"]
+        assert np.array_equal(Qb[:, b.offsets[p]:b.offsets[p + 1]], Qs), f"problem {p}"
+
+
+def test_batch_matches_oracle(ora):
+    from hyperelasticsolver_amd.solver import materials_array
+    print("[batch] vs oracle", flush=True)
+    ncells = [64, 150, 333]
+    steps = 12
+    for flux, fk in (("rusanov", 0), ("hll", 1)):
+        b, eos, sts, ifs = _build(ora, ncells, flux=flux, seed=2)
+        Q0 = b.get_state()
+        b.run(steps, mode="eager")
+        Qb = b.get_state()
+        b.close()
+        for p, n in enumerate(ncells):
+            Qo, to, so, _ = ora.run(1, materials_array(eos[p]), fk,
+                                    Q0[:, b.offsets[p]:b.offsets[p + 1]], 0.6, 1.0 / n, steps=steps)
+            err = rel_err_per_component(Qb[:, b.offsets[p]:b.offsets[p + 1]], Qo)
+            assert np.all(err <= TOL), (flux, p, err.max())
+
+
+def test_batch_two_phase_matches_oracle(ora):
+    from hyperelasticsolver_amd.solver import materials_array
+    print("[batch] two-phase vs oracle", flush=True)
+    ncells = [40, 70]
+    b, eos, sts, ifs = _build(ora, ncells, nph=2, flux="hll", seed=3)
+    Q0 = b.get_state()
+    b.run(5, mode="graph")
+    Qb = b.get_state()
+    b.close()
+    for p, n in enumerate(ncells):
+        Qo, _, _, _ = ora.run(2, materials_array(eos[p]), 1, Q0[:, b.offsets[p]:b.offsets[p + 1]],
+                              0.6, 1.0 / n, steps=5)
+        err = rel_err_per_component(Qb[:, b.offsets[p]:b.offsets[p + 1]], Qo)
+        assert np.all(err <= TOL), (p, err.max())
+
+
+def test_batch_modes_agree_and_autotune_restores(ora):
+    print("[batch] modes + autotune", flush=True)
+    ncells = [70, 5, 300, 1000, 62]
+    res = []
+    for mode in ("eager", "graph", "loop"):
+        b, *_ = _build(ora, ncells, seed=4)
+        if mode == "graph":
+            tuned = b.autotune(steps=2)
+            assert tuned["block"] in (64, 256)
+        b.run(9, mode=mode)
+        res.append((b.get_state(), b.times()))
+        b.close()
+    for Q, (t, s) in res[1:]:
+        assert np.array_equal(Q, res[0][0])
+        assert np.array_equal(t, res[0][1][0])
+        assert np.array_equal(s, res[0][1][1])
+
+
+def test_batch_per_problem_t_end(ora):
+    print("[batch] per-problem t_end", flush=True)
+    ncells = [100, 200]
+    b, eos, *_ = _build(ora, ncells, seed=5)
+    b.set_t_end([0.002, 0.004])
+    b.run(400, mode="graph")
+    t, s = b.times()
+    b.close()
+    assert t[0] >= 0.002 and t[1] >= 0.004
+    assert s[0] < s[1] < 400
