@@ -70,6 +70,47 @@ def cpu_baseline(eos, Qs, ifc, flux_kind, threads):
                       f" OpenMP over cells, {el:.1f} s wall"}
 
 
+def batched_problem(seed=4):
+    """4096 problems, lengths floor(exp(U(ln 256, ln 16384))), (rho0, c0, b0)
+    scaled by U(0.8, 1.2), 1-8 seeded interfaces each, single phase."""
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    from hyperelasticsolver_amd.initial import prim2cons, random_interfaces, random_phase_prims
+    rng = np.random.default_rng(seed)
+    P = 4096
+    ncells = np.floor(np.exp(rng.uniform(np.log(256), np.log(16384), P))).astype(np.int64)
+    eos, states, ifcs = [], [], []
+    for p in range(P):
+        s = rng.uniform(0.8, 1.2, 3)
+        e = Barton2009(_rho0=8.93 * s[0], _c0=4.6 * s[1], _b0=2.1 * s[2])
+        k = int(rng.integers(1, 9)) + 1
+        eos.append(e)
+        states.append(prim2cons(e, random_phase_prims(rng, e, k)))
+        ifcs.append(random_interfaces(rng, k - 1))
+    return eos, ncells, states, ifcs
+
+
+def time_batched(steps, loop_steps=5):
+    from hyperelasticsolver_amd.batched import Batch
+    eos, ncells, states, ifcs = batched_problem()
+    total = int(ncells.sum())
+    with Batch(eos, ncells, flux="rusanov") as b:
+        b.set_piecewise(states, ifcs)
+        tuned = b.autotune(steps=3)
+        b.run(2, mode="graph")                     # warm-up
+        ms_graph = b.run(steps, mode="graph")       # whole loop in one hipGraph launch
+        ms_loop = b.run(loop_steps, mode="loop")    # one launch per problem per step
+        Q = b.get_state()
+    if not np.all(np.isfinite(Q)):
+        raise RuntimeError("non-finite batched state")
+    per_step = ms_graph / steps
+    loop_per_step = ms_loop / loop_steps
+    return {"value": total * steps / (ms_graph / 1e3), "unit": "cell-updates/s",
+            "problems": len(ncells), "cells": total, "steps": steps, "ms_per_step": per_step,
+            "hbm_frac": BYTES_PER_CELL_1PH * total / (per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "loop_ms_per_step": loop_per_step, "speedup_vs_one_problem_per_launch": loop_per_step / per_step,
+            "autotune": tuned}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,7 +119,8 @@ def main():
     ap.add_argument("--nx", type=int, default=1 << 22)
     ap.add_argument("--flux", default="rusanov", choices=["rusanov", "hll"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the HLL line")
+    ap.add_argument("--no-extra", action="store_true", help="skip the HLL and batched lines")
+    ap.add_argument("--batched-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU slab "
@@ -121,6 +163,7 @@ def main():
         v2 = nx * args.steps / (ms2 / 1e3)
         out[other] = {"value": v2, "ms_per_step": ms2 / args.steps,
                       "hbm_frac": BYTES_PER_CELL_1PH * v2 / 1e9 / HBM_PEAK_GBS}
+        out["batched"] = time_batched(args.batched_steps)
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
         out["cpu_baseline"] = cpu_baseline(eos, Qs, ifc, 0 if args.flux == "rusanov" else 1, threads)

@@ -161,3 +161,30 @@ def test_parity_many_blocks(ora, flux):
     print(f"  max per-component rel err {err.max():.3e}", flush=True)
     assert np.all(err <= TOL), err
     np.testing.assert_allclose(r["dt"], dto, rtol=TOL, atol=0)
+
+
+@pytest.mark.parametrize("name,flux", [("config1_rusanov", "rusanov"), ("config1_hll", "hll")])
+def test_golden_config1_full_run(name, flux):
+    """1000-cell Riemann problem run to t_end = 0.06 (701 steps) against the
+    committed oracle fixture: 1e-10 relative per conserved variable after the
+    full run, same step count."""
+    import os
+    print(f"[golden] {name}", flush=True)
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
+    Q, r = _gpu_run(1, g["mat"], flux, g["Q0"], None, use_graph=True, t_end=0.06)
+    assert r["steps"] == int(g["steps"])
+    err = rel_err_per_component(Q, g["Q"])
+    print(f"  steps {r['steps']} max per-component rel err {err.max():.3e}", flush=True)
+    assert np.all(err <= TOL), err
+    assert abs(r["t"] - float(g["t"])) <= TOL * float(g["t"])
+
+
+def test_golden_testcase6():
+    """HyperelasticityMPh test case 6 (main.jl's own two-phase case) with HLL."""
+    import os
+    print("[golden] testcase6_hll", flush=True)
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "testcase6_hll.npz"))
+    Q, r = _gpu_run(2, g["mat"], "hll", g["Q0"], int(g["steps"]))
+    err = rel_err_per_component(Q, g["Q"])
+    print(f"  max per-component rel err {err.max():.3e}", flush=True)
+    assert np.all(err <= TOL), err

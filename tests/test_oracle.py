@@ -1,0 +1,393 @@
+"""Pins the CPU oracle (oracle/hes_oracle.cpp) without the reference.
+
+The reference (Julia) cannot run in this image, so every function of the oracle
+is checked against something its own code does not share: numpy/LAPACK for the
+3x3 algebra and eigenvalues, closed forms derived independently of the dual
+number path, finite differences, exact physical identities and the reference's
+own known answers (uniform states, identical phases at a = 1/2).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle_bind import rel_err_per_component
+
+MAT = np.array([8.93, 4.6, 3.9e-4, 300.0, 2.1, 1.0, 3.0, 2.0])       # Barton2009() (EquationsOfState.jl:90)
+MAT_ALT = np.array([8.93, 6.22, 9.0e-4, 300.0, 3.16, 1.0, 3.577, 2.088])  # main.jl:133 alternative
+MAT2 = np.concatenate([MAT, [2.7, 4.6, 3.9e-4, 300.0, 3.1, 1.0, 3.0, 2.0]])
+
+
+def vec(M):
+    return np.asarray(M, dtype=float).reshape(-1, order="F").copy()
+
+
+def mat(v):
+    return np.asarray(v, dtype=float).reshape(3, 3, order="F")
+
+
+def rand_A(rng, scale=0.05):
+    while True:
+        A = np.eye(3) + scale * rng.standard_normal((3, 3))
+        if np.linalg.det(A) > 0.3:
+            return A
+
+
+def rot(rng):
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+# --------------------------------------------------------------------------- SimpleLA / Strains
+def test_inv_det_against_numpy(ora):
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        M = rng.standard_normal((3, 3))
+        np.testing.assert_allclose(mat(ora.inv(vec(M))), np.linalg.inv(M), rtol=1e-12, atol=1e-12)
+        assert abs(ora.det(vec(M)) - np.linalg.det(M)) <= 1e-13 * max(1, abs(np.linalg.det(M)))
+        assert abs(ora.det_lu(vec(M)) - np.linalg.det(M)) <= 1e-13 * max(1, abs(np.linalg.det(M)))
+
+
+def test_finger_and_invariants(ora):
+    rng = np.random.default_rng(1)
+    for _ in range(30):
+        A = rand_A(rng, 0.2)
+        G = mat(ora.finger(vec(A)))
+        np.testing.assert_allclose(G, np.linalg.inv(A @ A.T), rtol=1e-12, atol=1e-13)
+        lam = np.linalg.eigvalsh(G)  # invariants are the elementary symmetric functions
+        i = ora.invariants(vec(G))
+        np.testing.assert_allclose(i, [lam.sum(), lam[0] * lam[1] + lam[1] * lam[2] + lam[0] * lam[2],
+                                       lam.prod()], rtol=1e-12)
+
+
+# --------------------------------------------------------------------------- energy / entropy
+def _energy_eig(m, S, G):
+    """Energy written through the eigenvalues of G: I1^2/3 - I2 = 1/6 sum_{i<j}(l_i - l_j)^2."""
+    rho0, c0, cv, t0, b0, al, be, ga = m
+    k0 = c0 ** 2 - 4.0 / 3.0 * b0 ** 2
+    lam = np.linalg.eigvalsh(G)
+    i3 = lam.prod()
+    dev = ((lam[0] - lam[1]) ** 2 + (lam[1] - lam[2]) ** 2 + (lam[0] - lam[2]) ** 2) / 6.0
+    return (0.5 * k0 / al ** 2 * (i3 ** (al / 2) - 1) ** 2 + cv * t0 * i3 ** (ga / 2) * (math.exp(S / cv) - 1)
+            + 0.5 * b0 ** 2 * i3 ** (be / 2) * dev)
+
+
+@pytest.mark.parametrize("m", [MAT, MAT_ALT])
+def test_energy_known_values(ora, m):
+    assert ora.energy(m, 0.0, vec(np.eye(3))) == 0.0  # undeformed, S = 0
+    rng = np.random.default_rng(2)
+    for _ in range(30):
+        A = rand_A(rng, 0.1)
+        S = rng.uniform(-1e-3, 2e-3)
+        G = np.linalg.inv(A @ A.T)
+        e = ora.energy(m, S, vec(G))
+        assert abs(e - _energy_eig(m, S, G)) <= 1e-11 * max(1.0, abs(e))
+        # rotation invariance of an isotropic energy
+        R = rot(rng)
+        assert abs(ora.energy(m, S, vec(R @ G @ R.T)) - e) <= 1e-11 * max(1.0, abs(e))
+
+
+def test_entropy_inverts_energy_and_clamps(ora):
+    rng = np.random.default_rng(3)
+    for _ in range(30):
+        G = np.linalg.inv(rand_A(rng, 0.1) @ rand_A(rng, 0.0).T)
+        S = rng.uniform(-1e-3, 3e-3)
+        e = ora.energy(MAT, S, vec(G))
+        assert abs(ora.entropy(MAT, e, vec(G)) - S) <= 1e-9 * max(1e-3, abs(S))
+    # far below the cold curve the logarithm's argument is clamped at 1e-6
+    G = vec(np.eye(3))
+    assert ora.entropy(MAT, -1.0, G) == pytest.approx(MAT[2] * math.log(1e-6), rel=1e-15)
+
+
+def test_temperature_is_de_dS(ora):
+    rng = np.random.default_rng(4)
+    for _ in range(10):
+        G = vec(np.linalg.inv(rand_A(rng) @ rand_A(rng, 0.0).T))
+        S, h = 1e-3, 1e-8
+        fd = (ora.energy(MAT, S + h, G) - ora.energy(MAT, S - h, G)) / (2 * h)
+        assert abs(ora.temperature(MAT, S, G) - fd) <= 1e-6 * abs(fd)
+
+
+# --------------------------------------------------------------------------- stress
+def _stress_closed(m, S, A):
+    """sigma = -2 rho [kappa (G^2 - I1/3 G) + c3 I], derived by hand from the energy."""
+    rho0, c0, cv, t0, b0, al, be, ga = m
+    k0 = c0 ** 2 - 4.0 / 3.0 * b0 ** 2
+    rho = rho0 / np.linalg.det(A)
+    G = np.linalg.inv(A @ A.T)
+    i1 = np.trace(G)
+    i2 = 0.5 * (i1 ** 2 - np.trace(G @ G))
+    i3 = np.linalg.det(G)
+    sa, sb, sg = i3 ** (al / 2), i3 ** (be / 2), i3 ** (ga / 2)
+    W = 0.5 * b0 ** 2 * sb * (i1 ** 2 / 3 - i2)
+    Eth = cv * t0 * sg * (math.exp(S / cv) - 1)
+    c3 = k0 / (2 * al) * (sa - 1) * sa + ga / 2 * Eth + be / 2 * W
+    return -2 * rho * (0.5 * b0 ** 2 * sb * (G @ G - i1 / 3 * G) + c3 * np.eye(3))
+
+
+@pytest.mark.parametrize("m", [MAT, MAT_ALT])
+def test_stress(ora, m):
+    assert np.all(ora.stress(m, 0.0, vec(np.eye(3))) == 0.0)  # sigma(I, 0) = 0
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        A = rand_A(rng)
+        S = rng.uniform(0, 2e-3)
+        sig = mat(ora.stress(m, S, vec(A)))
+        ref = _stress_closed(m, S, A)
+        np.testing.assert_allclose(sig, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+        np.testing.assert_allclose(sig, sig.T, atol=1e-12 * np.abs(sig).max())
+        R = rot(rng)  # objectivity: sigma(R A) = R sigma(A) R^T
+        np.testing.assert_allclose(mat(ora.stress(m, S, vec(R @ A))), R @ sig @ R.T,
+                                   atol=1e-11 * np.abs(sig).max())
+
+
+def test_stress_is_gradient_of_energy(ora):
+    """-2 rho G dE/dG with dE/dG by central differences of the oracle's energy."""
+    rng = np.random.default_rng(6)
+    A = rand_A(rng)
+    S = 1e-3
+    G = np.linalg.inv(A @ A.T)
+    h = 1e-6
+    D = np.zeros((3, 3))
+    for k in range(3):
+        for l in range(3):
+            E = np.zeros((3, 3))
+            E[k, l] = h
+            D[k, l] = (ora.energy(MAT, S, vec(G + E)) - ora.energy(MAT, S, vec(G - E))) / (2 * h)
+    ref = -2 * (MAT[0] / np.linalg.det(A)) * G @ D
+    np.testing.assert_allclose(mat(ora.stress(MAT, S, vec(A))), ref, atol=1e-7 * np.abs(ref).max())
+
+
+def test_stress_den_form(ora):
+    """stress(eos, den, e_int, F) recomputes S from e_int and uses the given density."""
+    rng = np.random.default_rng(7)
+    A = rand_A(rng)
+    S = 1.5e-3
+    G = vec(np.linalg.inv(A @ A.T))
+    e = ora.energy(MAT, S, G)
+    den = 3.3
+    s1 = ora.stress_den(MAT, den, e, vec(A))
+    s2 = ora.stress(MAT, S, vec(A)) * (den / (MAT[0] / np.linalg.det(A)))
+    np.testing.assert_allclose(s1, s2, rtol=1e-9, atol=1e-12)
+
+
+# --------------------------------------------------------------------------- acoustic tensor / eigenvalues
+def test_acoustic_at_identity(ora):
+    ac = ora.acoustic(MAT, 0.0, vec(np.eye(3)))
+    np.testing.assert_allclose(ac, np.diag([MAT[1] ** 2, MAT[4] ** 2, MAT[4] ** 2]), atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [(1.0, 0.0, 0.0), (0.3, -0.5, 0.81)])
+def test_acoustic_is_stress_derivative(ora, n):
+    """Omega_ij = (1/rho) sum_m n_m d sigma(m,i) / dA along dA = e_j (A^T n)^T, by central differences."""
+    rng = np.random.default_rng(8)
+    n = np.asarray(n)
+    for _ in range(5):
+        A = rand_A(rng)
+        S = 1e-3
+        rho = MAT[0] / np.linalg.det(A)
+        h = 1e-6
+        ref = np.zeros((3, 3))
+        for j in range(3):
+            dA = np.zeros((3, 3))
+            dA[j, :] = A.T @ n
+            sp = mat(ora.stress(MAT, S, vec(A + h * dA)))
+            sm = mat(ora.stress(MAT, S, vec(A - h * dA)))
+            ref[:, j] = n @ (sp - sm) / (2 * h) / rho
+        ac = ora.acoustic(MAT, S, vec(A), n)
+        np.testing.assert_allclose(ac, ref, atol=2e-7 * np.abs(ref).max())
+        np.testing.assert_allclose(ac, ac.T, atol=1e-12 * np.abs(ac).max())
+
+
+def test_acoustic_prim_is_fixed_energy_derivative(ora):
+    """acoustic(eos, P, n): derivative of stress(eos, den, e_int, F) at fixed den and e_int."""
+    rng = np.random.default_rng(9)
+    A = rand_A(rng)
+    S = 1e-3
+    G = vec(np.linalg.inv(A @ A.T))
+    e = ora.energy(MAT, S, G)
+    den = 0.7 * 9.1
+    P = np.concatenate([[0.7, 9.1, 0.1, 0.2, 0.3, S], vec(A)])
+    h = 1e-6
+    ref = np.zeros((3, 3))
+    for j in range(3):
+        dA = np.zeros((3, 3))
+        dA[j, :] = A[0, :]
+        sp = mat(ora.stress_den(MAT, den, e, vec(A + h * dA)))
+        sm = mat(ora.stress_den(MAT, den, e, vec(A - h * dA)))
+        ref[:, j] = (sp - sm)[0, :] / (2 * h) / den
+    np.testing.assert_allclose(ora.acoustic_prim(MAT, P), ref, atol=2e-7 * np.abs(ref).max())
+
+
+def test_eigvals_against_lapack(ora):
+    rng = np.random.default_rng(10)
+    for _ in range(50):
+        M = rng.standard_normal((3, 3))
+        M = M + M.T
+        np.testing.assert_allclose(np.sort(ora.eigvals3(M)), np.sort(np.linalg.eigvals(M).real),
+                                   rtol=1e-12, atol=1e-12 * np.abs(M).max())
+    for _ in range(20):  # the acoustic tensors themselves (non-symmetric at round-off)
+        ac = ora.acoustic(MAT, 1e-3, vec(rand_A(rng, 0.1)))
+        np.testing.assert_allclose(np.sort(ora.eigvals3(ac)), np.sort(np.linalg.eigvals(ac).real),
+                                   rtol=1e-12)
+
+
+# --------------------------------------------------------------------------- model functions
+def _rand_prims(rng, nph, mats):
+    out = []
+    for p in range(nph):
+        A = rand_A(rng)
+        blk = [mats[8 * p] / np.linalg.det(A), *rng.uniform(-0.5, 0.5, 3), rng.uniform(0, 2e-3), *vec(A)]
+        out += ([rng.uniform(0.05, 0.95)] if nph == 2 else []) + blk
+    if nph == 2:
+        out[15] = 1 - out[0]
+    return np.array(out)
+
+
+@pytest.mark.parametrize("nph", [1, 2])
+def test_prim_cons_roundtrip_and_rho_ignored(ora, nph):
+    rng = np.random.default_rng(11)
+    mats = MAT if nph == 1 else MAT2
+    for _ in range(20):
+        P = _rand_prims(rng, nph, mats)
+        Q = ora.prim2cons(nph, mats, P)
+        np.testing.assert_allclose(ora.cons2prim(nph, mats, Q), P, rtol=1e-10, atol=1e-13)
+        # the conserved a*rho is never read (HyperelasticityMPh.jl:110-114)
+        Q2 = Q.copy()
+        Q2[0 if nph == 1 else 1] *= 1.7
+        assert np.array_equal(ora.cons2prim(nph, mats, Q2)[1:], ora.cons2prim(nph, mats, Q)[1:])
+        assert np.array_equal(ora.flux(nph, mats, Q2), ora.flux(nph, mats, Q))
+
+
+def test_flux_at_rest_is_minus_stress(ora):
+    rng = np.random.default_rng(12)
+    A = rand_A(rng)
+    P = np.concatenate([[MAT[0] / np.linalg.det(A), 0, 0, 0, 1e-3], vec(A)])
+    F = ora.flux(1, MAT, ora.prim2cons(1, MAT, P))
+    sig = mat(ora.stress(MAT, 1e-3, vec(A)))
+    np.testing.assert_allclose(F[1:4], -sig[0, :], rtol=1e-10)
+    assert F[0] == 0 and F[4] == 0 and np.all(F[5:] == 0)
+
+
+def test_noncons_structure(ora):
+    rng = np.random.default_rng(13)
+    Q = ora.prim2cons(2, MAT2, _rand_prims(rng, 2, MAT2))
+    B = ora.noncons_flux(MAT2, Q)
+    cols = [c for c in range(30) if np.any(B[:, c] != 0)]
+    assert cols == [0, 15]  # one non-zero column per phase block (the a_p column)
+    assert np.all(B[15:, 0] == 0) and np.all(B[:15, 15] == 0)
+    assert B[1, 0] == 0 and B[16, 15] == 0
+    # interface velocity/stress rows are common to both phases, so with
+    # da_2 = -da_1 the mixture momentum and energy contributions cancel
+    np.testing.assert_array_equal(B[[0, 2, 3, 4, 5], 0], B[[15, 17, 18, 19, 20], 15])
+
+
+def test_fluxes_of_equal_states(ora):
+    rng = np.random.default_rng(14)
+    for nph, mats in ((1, MAT), (2, MAT2)):
+        Q = ora.prim2cons(nph, mats, _rand_prims(rng, nph, mats))
+        c, dm, dp = ora.lxf(nph, mats, Q, Q, 7.0)
+        np.testing.assert_array_equal(c, ora.flux(nph, mats, Q))
+        assert np.all(dm == 0) and np.all(dp == 0)
+        e = ora.get_eigvals(nph, mats, Q)
+        c, dm, dp = ora.hll(nph, mats, Q, Q, e, e)
+        assert np.all(c == 0) and np.all(dm == 0) and np.all(dp == 0)
+
+
+def test_hll_reduces_to_textbook_hll(ora):
+    """With no non-conservative term, D- = F_hll - F_l and D+ = F_r - F_hll with the
+    Harten-Lax-van Leer flux F_hll = (s_r F_l - s_l F_r + s_l s_r (Q_r - Q_l)) / (s_r - s_l)."""
+    rng = np.random.default_rng(15)
+    for _ in range(10):
+        Ql = ora.prim2cons(1, MAT, _rand_prims(rng, 1, MAT))
+        Qr = ora.prim2cons(1, MAT, _rand_prims(rng, 1, MAT))
+        el, er = ora.get_eigvals(1, MAT, Ql), ora.get_eigvals(1, MAT, Qr)
+        em = ora.get_eigvals(1, MAT, 0.5 * (Ql + Qr))
+        sl = min(0.0, em.min(), el.min())
+        sr = max(0.0, em.max(), er.max())
+        Fl, Fr = ora.flux(1, MAT, Ql), ora.flux(1, MAT, Qr)
+        Fh = (sr * Fl - sl * Fr + sl * sr * (Qr - Ql)) / (sr - sl)
+        _, dm, dp = ora.hll(1, MAT, Ql, Qr, el, er)
+        scale = np.abs(Fl).max() + np.abs(Fr).max()
+        np.testing.assert_allclose(dm, Fh - Fl, atol=1e-13 * scale)
+        np.testing.assert_allclose(dp, Fr - Fh, atol=1e-13 * scale)
+
+
+def test_quadrature_rules(ora):
+    x, w = ora.quadrature(0)
+    xr, wr = np.polynomial.legendre.leggauss(6)
+    np.testing.assert_allclose(x, xr, atol=2e-16)
+    np.testing.assert_allclose(w, wr, atol=2e-16)
+    import mpmath
+    mpmath.mp.dps = 40
+    s7 = mpmath.sqrt(7)
+    lo = [-1, -mpmath.sqrt(mpmath.mpf(1) / 3 + 2 * s7 / 21), -mpmath.sqrt(mpmath.mpf(1) / 3 - 2 * s7 / 21)]
+    lo = lo + [-v for v in reversed(lo)]
+    lw = [mpmath.mpf(1) / 15, (14 - s7) / 30, (14 + s7) / 30]
+    lw = lw + list(reversed(lw))
+    x, w = ora.quadrature(1)
+    np.testing.assert_allclose(x, [float(v) for v in lo], atol=2e-16)
+    np.testing.assert_allclose(w, [float(v) for v in lw], atol=2e-16)
+    # Lobatto-6 integrates polynomials up to degree 9 exactly
+    assert abs(sum(wi * xi ** 8 for xi, wi in zip(x, w)) - 2 / 9) < 1e-15
+
+
+# --------------------------------------------------------------------------- time loop
+def _riemann(ora, nph, mats, nx, seed):
+    rng = np.random.default_rng(seed)
+    Ql = ora.prim2cons(nph, mats, _rand_prims(rng, nph, mats))
+    Qr = ora.prim2cons(nph, mats, _rand_prims(rng, nph, mats))
+    return np.asfortranarray(np.stack([Ql if i < nx / 2 else Qr for i in range(nx)], axis=1))
+
+
+@pytest.mark.parametrize("nph,flux", [(1, 0), (1, 1), (2, 0), (2, 1)])
+def test_uniform_state_stays_bit_identical(ora, nph, flux):
+    mats = MAT if nph == 1 else MAT2
+    rng = np.random.default_rng(16)
+    Q = ora.prim2cons(nph, mats, _rand_prims(rng, nph, mats))
+    Q0 = np.asfortranarray(np.repeat(Q[:, None], 20, axis=1))
+    Q1, t, s, dt = ora.run(nph, mats, flux, Q0, 0.6, 0.05, steps=5)
+    assert s == 5 and np.array_equal(Q1, Q0)
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+def test_identical_phases_at_half_are_half_the_single_phase_run(ora, flux):
+    """HyperelasticityMPh test cases 4/5 replicate the single-phase cases with two
+    identical phases at a = 1/2: every non-conservative term is multiplied by a
+    zero volume-fraction jump and everything else scales by 1/2 exactly."""
+    nx = 40
+    Q1 = _riemann(ora, 1, MAT, nx, seed=17)
+    Q2 = np.zeros((30, nx), order="F")
+    for p in range(2):
+        Q2[15 * p] = 0.5
+        Q2[15 * p + 1:15 * p + 15] = 0.5 * Q1
+    mats2 = np.concatenate([MAT, MAT])
+    R1, t1, s1, d1 = ora.run(1, MAT, flux, Q1, 0.6, 1.0 / nx, steps=8)
+    R2, t2, s2, d2 = ora.run(2, mats2, flux, Q2, 0.6, 1.0 / nx, steps=8)
+    assert np.array_equal(d1, d2)
+    for p in range(2):
+        assert np.all(R2[15 * p] == 0.5)
+        assert np.array_equal(R2[15 * p + 1:15 * p + 15], 0.5 * R1)
+
+
+def test_conservation_and_t_end(ora):
+    nx = 60
+    Q0 = _riemann(ora, 1, MAT, nx, seed=18)
+    Q0[:, -5:] = Q0[:, :5]  # same state at both frozen ends
+    Q1, t, s, dt = ora.run(1, MAT, 1, Q0, 0.6, 1.0 / nx, t_end=0.002)
+    assert t >= 0.002 and t - dt[-1] < 0.002  # `while t < T` (main.jl:202)
+    assert t == pytest.approx(dt.sum(), rel=1e-14)
+    err = np.abs(Q1.sum(axis=1) - Q0.sum(axis=1)) / np.abs(Q0).sum(axis=1)
+    assert np.all(err < 1e-14)
+
+
+def test_oracle_reproduces_golden_fixtures(ora):
+    """The committed fixtures were written by tests/golden/make_golden.py."""
+    import os
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    g = np.load(os.path.join(here, "testcase6_hll.npz"))
+    Q, t, s, dt = ora.run(2, g["mat"], 1, g["Q0"], 0.6, 1.0 / g["Q0"].shape[1], steps=int(g["steps"]))
+    assert np.array_equal(Q, g["Q"]) and t == float(g["t"])
+    g = np.load(os.path.join(here, "config1_rusanov.npz"))
+    Q, t, s, dt = ora.run(1, g["mat"], 0, g["Q0"], 0.6, 1.0 / g["Q0"].shape[1], t_end=0.06)
+    assert s == int(g["steps"]) and np.array_equal(Q, g["Q"])
