@@ -272,7 +272,7 @@ int batch_init_speeds(hes_batch* h) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
   int st = 0;
-  HIP_TRY(hipMemcpy(&st, h->status, sizeof(int), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_on(h->stream, &st, h->status, sizeof(int), hipMemcpyDeviceToHost));
   if (st) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
   return HES_OK;
 }
@@ -280,9 +280,9 @@ int batch_init_speeds(hes_batch* h) {
 int batch_after_set(hes_batch* h, const double* t0) {
   std::vector<double> tr(2 * h->P);
   for (int p = 0; p < h->P; ++p) tr[2 * p] = tr[2 * p + 1] = t0 ? t0[p] : 0.0;
-  HIP_TRY(hipMemcpy(h->tring, tr.data(), sizeof(double) * 2 * h->P, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemset(h->psteps, 0, sizeof(int) * h->P));
-  HIP_TRY(hipMemset(h->status, 0, sizeof(int) * 4));
+  HIP_TRY(copy_on(h->stream, h->tring, tr.data(), sizeof(double) * 2 * h->P, hipMemcpyHostToDevice));
+  HIP_TRY(zero_on(h->stream, h->psteps, sizeof(int) * h->P));
+  HIP_TRY(zero_on(h->stream, h->status, sizeof(int) * 4));
   h->step = 0;
   h->has_state = true;
   return batch_init_speeds(h);
@@ -351,14 +351,14 @@ int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const
   h->N = h->offsets_h[nproblems];
   h->pitch = (h->N + 63) / 64 * 64;
   size_t qb = sizeof(double) * (size_t)h->nc * h->pitch, cb = sizeof(double) * (size_t)nphase * h->pitch;
-  hipError_t e = hipSuccess;
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   for (int b = 0; b < 2 && e == hipSuccess; ++b) {
     e = hipMalloc(&h->Q[b], qb);
     if (e == hipSuccess) e = hipMalloc(&h->C[b], cb);
   }
   auto up = [&](void** d, const void* src, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(d, bytes);
-    if (e == hipSuccess && src) e = hipMemcpy(*d, src, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess && src) e = copy_on(h->stream, *d, src, bytes, hipMemcpyHostToDevice);
   };
   up((void**)&h->tile_start, tstart.data(), sizeof(long long) * tstart.size());
   up((void**)&h->tile_prob, tprob.data(), sizeof(int) * tprob.size());
@@ -371,7 +371,6 @@ int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const
   up((void**)&h->slots, nullptr, sizeof(unsigned long long) * 3 * nproblems);
   up((void**)&h->psteps, nullptr, sizeof(int) * nproblems);
   up((void**)&h->status, nullptr, sizeof(int) * 4);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);
   if (e == hipSuccess) e = hipEventCreate(&h->ev1);
   if (e != hipSuccess) {
@@ -397,7 +396,7 @@ int hes_batch_set_t_end(hes_batch* h, const double* t_end) {
   std::vector<double> v(h->P, INFINITY);
   if (t_end)
     for (int p = 0; p < h->P; ++p) v[p] = t_end[p];
-  HIP_TRY(hipMemcpy(h->t_end, v.data(), sizeof(double) * h->P, hipMemcpyHostToDevice));
+  HIP_TRY(copy_on(h->stream, h->t_end, v.data(), sizeof(double) * h->P, hipMemcpyHostToDevice));
   return HES_OK;
 }
 
@@ -406,7 +405,7 @@ int hes_batch_set_state(hes_batch* h, const double* Q, const double* t0) {
   size_t bytes = sizeof(double) * (size_t)h->nc * h->N;
   double* tmp = nullptr;
   HIP_TRY(hipMalloc(&tmp, bytes));
-  hipError_t e = hipMemcpy(tmp, Q, bytes, hipMemcpyHostToDevice);
+  hipError_t e = copy_on(h->stream, tmp, Q, bytes, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_aos_to_soa_b, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->stream,
                        tmp, h->Q[0], h->nc, h->N, h->pitch);
@@ -439,9 +438,12 @@ int hes_batch_set_piecewise(hes_batch* h, const int* nstates, const double* stat
   HIP_TRY(hipMalloc(&dso, sizeof(int) * (h->P + 1)));
   hipError_t e = hipMalloc(&dst, sizeof(double) * (size_t)S * h->nc);
   if (e == hipSuccess) e = hipMalloc(&dif, sizeof(double) * (size_t)(I > 0 ? I : 1));
-  if (e == hipSuccess) e = hipMemcpy(dso, soff.data(), sizeof(int) * (h->P + 1), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(dst, states, sizeof(double) * (size_t)S * h->nc, hipMemcpyHostToDevice);
-  if (e == hipSuccess && I > 0) e = hipMemcpy(dif, interfaces, sizeof(double) * I, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = copy_on(h->stream, dso, soff.data(), sizeof(int) * (h->P + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = copy_on(h->stream, dst, states, sizeof(double) * (size_t)S * h->nc, hipMemcpyHostToDevice);
+  if (e == hipSuccess && I > 0)
+    e = copy_on(h->stream, dif, interfaces, sizeof(double) * I, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_batch_piecewise, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0,
                        h->stream, h->offsets, h->P, dso, dst, dif, h->nc, h->Q[0], h->N, h->pitch);
@@ -481,8 +483,8 @@ int hes_batch_get_times(hes_batch* h, double* t, long long* steps) {
   if (!h) return fail(HES_ERR_ARG, "null handle");
   std::vector<double> tr(2 * h->P);
   std::vector<int> ps(h->P);
-  HIP_TRY(hipMemcpy(tr.data(), h->tring, sizeof(double) * 2 * h->P, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(ps.data(), h->psteps, sizeof(int) * h->P, hipMemcpyDeviceToHost));
+  HIP_TRY(copy_on(h->stream, tr.data(), h->tring, sizeof(double) * 2 * h->P, hipMemcpyDeviceToHost));
+  HIP_TRY(copy_on(h->stream, ps.data(), h->psteps, sizeof(int) * h->P, hipMemcpyDeviceToHost));
   for (int p = 0; p < h->P; ++p) {
     if (t) t[p] = tr[2 * p + (h->step & 1)];
     if (steps) steps[p] = ps[p];
@@ -546,7 +548,7 @@ int hes_batch_run(hes_batch* h, long long steps, int mode, float* elapsed_ms) {
   if (elapsed_ms) *elapsed_ms = ms;
   h->step += steps;
   int st = 0;
-  HIP_TRY(hipMemcpy(&st, h->status, sizeof(int), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_on(h->stream, &st, h->status, sizeof(int), hipMemcpyDeviceToHost));
   if (st) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
   return HES_OK;
 }
@@ -566,11 +568,12 @@ int hes_batch_autotune(hes_batch* h, int steps, int* best_block, int* best_xcd, 
   HIP_TRY(hipMalloc(&t0, sizeof(double) * 2 * h->P));
   HIP_TRY(hipMalloc(&sl0, sizeof(unsigned long long) * 3 * h->P));
   HIP_TRY(hipMalloc(&ps0, sizeof(int) * h->P));
-  HIP_TRY(hipMemcpy(q0, h->Q[s0 & 1], qb, hipMemcpyDeviceToDevice));
-  HIP_TRY(hipMemcpy(c0, h->C[s0 & 1], cb, hipMemcpyDeviceToDevice));
-  HIP_TRY(hipMemcpy(t0, h->tring, sizeof(double) * 2 * h->P, hipMemcpyDeviceToDevice));
-  HIP_TRY(hipMemcpy(sl0, h->slots, sizeof(unsigned long long) * 3 * h->P, hipMemcpyDeviceToDevice));
-  HIP_TRY(hipMemcpy(ps0, h->psteps, sizeof(int) * h->P, hipMemcpyDeviceToDevice));
+  const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+  HIP_TRY(copy_on(h->stream, q0, h->Q[s0 & 1], qb, dd));
+  HIP_TRY(copy_on(h->stream, c0, h->C[s0 & 1], cb, dd));
+  HIP_TRY(copy_on(h->stream, t0, h->tring, sizeof(double) * 2 * h->P, dd));
+  HIP_TRY(copy_on(h->stream, sl0, h->slots, sizeof(unsigned long long) * 3 * h->P, dd));
+  HIP_TRY(copy_on(h->stream, ps0, h->psteps, sizeof(int) * h->P, dd));
   const int blocks[2] = {64, 256};
   float best = 1e30f;
   int bb = h->block, bx = h->xcd, rc = HES_OK;
@@ -584,11 +587,11 @@ int hes_batch_autotune(hes_batch* h, int steps, int* best_block, int* best_xcd, 
       if (rc == HES_OK && ms < best) { best = ms; bb = blocks[bi]; bx = x; }
       // restore the starting state for the next candidate
       h->step = s0;
-      if (hipMemcpy(h->Q[s0 & 1], q0, qb, hipMemcpyDeviceToDevice) != hipSuccess ||
-          hipMemcpy(h->C[s0 & 1], c0, cb, hipMemcpyDeviceToDevice) != hipSuccess ||
-          hipMemcpy(h->tring, t0, sizeof(double) * 2 * h->P, hipMemcpyDeviceToDevice) != hipSuccess ||
-          hipMemcpy(h->slots, sl0, sizeof(unsigned long long) * 3 * h->P, hipMemcpyDeviceToDevice) != hipSuccess ||
-          hipMemcpy(h->psteps, ps0, sizeof(int) * h->P, hipMemcpyDeviceToDevice) != hipSuccess)
+      if (copy_on(h->stream, h->Q[s0 & 1], q0, qb, dd) != hipSuccess ||
+          copy_on(h->stream, h->C[s0 & 1], c0, cb, dd) != hipSuccess ||
+          copy_on(h->stream, h->tring, t0, sizeof(double) * 2 * h->P, dd) != hipSuccess ||
+          copy_on(h->stream, h->slots, sl0, sizeof(unsigned long long) * 3 * h->P, dd) != hipSuccess ||
+          copy_on(h->stream, h->psteps, ps0, sizeof(int) * h->P, dd) != hipSuccess)
         rc = fail(HES_ERR_HIP, "autotune restore failed");
     }
   (void)hipFree(q0);

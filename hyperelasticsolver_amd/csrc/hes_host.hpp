@@ -62,4 +62,20 @@ inline bool mat_ok(const double* p) {
   return p[0] > 0 && p[2] > 0 && p[3] > 0 && p[5] != 0;
 }
 
+// Copies and fills that belong to a handle run on the handle's stream and are
+// waited for: a plain hipMemcpy / hipMemset goes to the null stream, which is
+// not ordered with the handle's non-blocking stream.
+inline hipError_t copy_on(hipStream_t s, void* dst, const void* src, size_t bytes,
+                          hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e;
+}
+
+inline hipError_t zero_on(hipStream_t s, void* dst, size_t bytes) {
+  hipError_t e = hipMemsetAsync(dst, 0, bytes, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e;
+}
+
 }  // namespace hes

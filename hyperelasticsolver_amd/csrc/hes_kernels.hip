@@ -503,7 +503,7 @@ int init_speeds(hes_solver* h) {
   }
   HIP_TRY(hipStreamSynchronize(h->stream));
   int st[2];
-  HIP_TRY(hipMemcpy(st, h->status, sizeof(st), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_on(h->stream, st, h->status, sizeof(st), hipMemcpyDeviceToHost));
   if (st[0]) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
   return HES_OK;
 }
@@ -565,7 +565,7 @@ int hes_solver_create(hes_solver** out, int nphase, int flux, long long ncells,
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);
   if (e == hipSuccess) e = hipEventCreate(&h->ev1);
-  if (e == hipSuccess) e = hipMemset(h->status, 0, 4 * sizeof(int));
+  if (e == hipSuccess) e = zero_on(h->stream, h->status, 4 * sizeof(int));
   if (e != hipSuccess) {
     g_err = std::string("allocation failed: ") + hipGetErrorString(e);
     hes_solver_destroy(h);
@@ -612,7 +612,7 @@ static int set_state_common(hes_solver* h, const double* src, hipMemcpyKind kind
     g_err = std::string("set_state: ") + hipGetErrorString(e);
     return HES_ERR_HIP;
   }
-  HIP_TRY(hipMemset(h->status, 0, 4 * sizeof(int)));
+  HIP_TRY(zero_on(h->stream, h->status, 4 * sizeof(int)));
   h->has_state = true;
   return init_speeds(h);
 }
@@ -631,9 +631,11 @@ int hes_solver_set_piecewise(hes_solver* h, int nstates, const double* states,
   double *ds = nullptr, *di = nullptr;
   HIP_TRY(hipMalloc(&ds, sizeof(double) * (size_t)nstates * h->nc));
   HIP_TRY(hipMalloc(&di, sizeof(double) * (size_t)(nstates > 1 ? nstates - 1 : 1)));
-  hipError_t e = hipMemcpy(ds, states, sizeof(double) * (size_t)nstates * h->nc, hipMemcpyHostToDevice);
+  hipError_t e = copy_on(h->stream, ds, states, sizeof(double) * (size_t)nstates * h->nc,
+                          hipMemcpyHostToDevice);
   if (e == hipSuccess && nstates > 1)
-    e = hipMemcpy(di, interfaces, sizeof(double) * (size_t)(nstates - 1), hipMemcpyHostToDevice);
+    e = copy_on(h->stream, di, interfaces, sizeof(double) * (size_t)(nstates - 1),
+                hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     h->step = 0;
     h->t = t0;
@@ -649,7 +651,7 @@ int hes_solver_set_piecewise(hes_solver* h, int nstates, const double* states,
     g_err = std::string("set_piecewise: ") + hipGetErrorString(e);
     return HES_ERR_HIP;
   }
-  HIP_TRY(hipMemset(h->status, 0, 4 * sizeof(int)));
+  HIP_TRY(zero_on(h->stream, h->status, 4 * sizeof(int)));
   h->has_state = true;
   return init_speeds(h);
 }
@@ -682,8 +684,8 @@ int hes_solver_get_state(hes_solver* h, double* Q) {
 int hes_solver_get_speeds(hes_solver* h, double* c) {
   if (!h || !c) return fail(HES_ERR_ARG, "null argument");
   for (int p = 0; p < h->nph; ++p)
-    HIP_TRY(hipMemcpy(c + p * h->n, h->C[h->step & 1] + p * h->pitch, sizeof(double) * h->n,
-                      hipMemcpyDeviceToHost));
+    HIP_TRY(copy_on(h->stream, c + p * h->n, h->C[h->step & 1] + p * h->pitch,
+                    sizeof(double) * h->n, hipMemcpyDeviceToHost));
   return HES_OK;
 }
 
@@ -697,7 +699,7 @@ int hes_solver_time(hes_solver* h, double* t, long long* steps) {
 int hes_solver_next_dt(hes_solver* h, double* dt) {
   if (!h || !dt) return fail(HES_ERR_ARG, "null argument");
   unsigned long long bits = 0;
-  HIP_TRY(hipMemcpy(&bits, h->slots + (h->step % 3), sizeof(bits), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_on(h->stream, &bits, h->slots + (h->step % 3), sizeof(bits), hipMemcpyDeviceToHost));
   double lmax;
   std::memcpy(&lmax, &bits, sizeof(lmax));
   *dt = h->cfl * h->dx / lmax;
@@ -772,17 +774,18 @@ int hes_solver_run(hes_solver* h, long long max_steps, double t_end, int use_gra
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   int st[2];
-  HIP_TRY(hipMemcpy(st, h->status, sizeof(st), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_on(h->stream, st, h->status, sizeof(st), hipMemcpyDeviceToHost));
   done = st[1];
   h->step = s0 + done;
   double tr[2];
-  HIP_TRY(hipMemcpy(tr, h->tring, sizeof(tr), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_on(h->stream, tr, h->tring, sizeof(tr), hipMemcpyDeviceToHost));
   h->t = tr[h->step & 1];
   if (steps_done) *steps_done = done;
   if (t_now) *t_now = h->t;
   if (elapsed_ms) *elapsed_ms = ms;
   if (dt_out && done > 0)
-    HIP_TRY(hipMemcpy(dt_out, h->dt_hist, sizeof(double) * (size_t)done, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_on(h->stream, dt_out, h->dt_hist, sizeof(double) * (size_t)done,
+                    hipMemcpyDeviceToHost));
   if (st[0]) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
   return HES_OK;
 }
@@ -800,10 +803,10 @@ int hes_solver_time_steps(hes_solver* h, long long reps, float* elapsed_ms) {
   HIP_TRY(hipEventSynchronize(h->ev1));
   HIP_TRY(hipEventElapsedTime(elapsed_ms, h->ev0, h->ev1));
   int st[2];
-  HIP_TRY(hipMemcpy(st, h->status, sizeof(st), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_on(h->stream, st, h->status, sizeof(st), hipMemcpyDeviceToHost));
   h->step += st[1];
   double tr[2];
-  HIP_TRY(hipMemcpy(tr, h->tring, sizeof(tr), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_on(h->stream, tr, h->tring, sizeof(tr), hipMemcpyDeviceToHost));
   h->t = tr[h->step & 1];
   if (st[0]) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
   return HES_OK;

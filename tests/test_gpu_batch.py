@@ -146,3 +146,25 @@ def test_batch_per_problem_t_end(ora):
     b.close()
     assert t[0] >= 0.002 and t[1] >= 0.004
     assert s[0] < s[1] < 400
+
+
+def test_autotune_on_reused_memory_matches_clean_run(ora):
+    """Device memory full of NaNs from an earlier allocation must never leak in:
+    the autotuner's save/restore copies are ordered with its kernels."""
+    import torch
+    print("[batch] autotune on NaN-filled reused memory", flush=True)
+    ncells = [300, 4000, 777, 5000] * 4
+    b, *_ = _build(ora, ncells, seed=6)
+    b.run(6, mode="eager")
+    clean = b.get_state()
+    b.close()
+    x = torch.full((1 << 29,), float("nan"), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    del x
+    torch.cuda.empty_cache()
+    b, *_ = _build(ora, ncells, seed=6)
+    b.autotune(steps=3)
+    b.set_launch(256, True)
+    b.run(6, mode="eager")
+    assert np.array_equal(b.get_state(), clean)
+    b.close()
