@@ -1,0 +1,10 @@
+// Host build of the generated per-cell primal/VJP routines, for the CPU tests.
+#include "hes_adjoint_gen.hpp"
+extern "C" {
+void gen_cell_fwd(const double* in, double* out) { hes_gen::cell_fwd(in, out); }
+void gen_cell_vjp(const double* in, const double* ob, double* ib) { hes_gen::cell_vjp(in, ob, ib); }
+void gen_p2c_fwd(const double* in, double* out) { hes_gen::prim2cons_fwd(in, out); }
+void gen_p2c_vjp(const double* in, const double* ob, double* ib) { hes_gen::prim2cons_vjp(in, ob, ib); }
+void gen_loss_fwd(const double* in, double* out) { hes_gen::loss_fwd(in, out); }
+void gen_loss_vjp(const double* in, const double* ob, double* ib) { hes_gen::loss_vjp(in, ob, ib); }
+}

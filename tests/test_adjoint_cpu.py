@@ -1,0 +1,115 @@
+"""The generated per-cell primal and reverse-mode routines (hes_adjoint_gen.hpp),
+compiled for the host: primal against the oracle, VJPs against central
+differences of the primal."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "hyperelasticsolver_amd", "csrc")
+MAT = np.array([8.93, 4.6, 3.9e-4, 300.0, 2.1, 1.0, 3.0, 2.0])
+
+
+@pytest.fixture(scope="module")
+def gen(tmp_path_factory):
+    so = str(tmp_path_factory.mktemp("gen") / "adjoint_host.so")
+    subprocess.run(["g++", "-O1", "-shared", "-fPIC", "-ffp-contract=off", "-I" + CSRC,
+                    os.path.join(ROOT, "tests", "adjoint_host.cpp"), "-o", so], check=True)
+    L = ctypes.CDLL(so)
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def call(fn, x, nout):
+    x = np.ascontiguousarray(x, dtype=float)
+    out = np.zeros(nout)
+    fn(_p(x), _p(out))
+    return out
+
+
+def vjp(fn, x, ob):
+    x = np.ascontiguousarray(x, dtype=float)
+    ob = np.ascontiguousarray(ob, dtype=float)
+    ib = np.zeros_like(x)
+    fn(_p(x), _p(ob), _p(ib))
+    return ib
+
+
+def rand_state(ora, rng):
+    A = np.eye(3) + 0.05 * rng.standard_normal((3, 3))
+    P = np.concatenate([[MAT[0] / np.linalg.det(A)], rng.uniform(-0.5, 0.5, 3), [1e-3],
+                        A.reshape(-1, order="F")])
+    return ora.prim2cons(1, MAT, P), P
+
+
+def cell_in(Q, m=MAT):
+    return np.concatenate([Q, [m[4], m[1], m[0], m[2], m[3]]])  # q, b0, c0, rho0, cv, t0
+
+
+def test_cell_primal_matches_oracle(gen, ora):
+    rng = np.random.default_rng(0)
+    for _ in range(10):
+        Q, P = rand_state(ora, rng)
+        out = call(gen.gen_cell_fwd, cell_in(Q), 21)
+        F = ora.flux(1, MAT, Q)
+        np.testing.assert_allclose(out[:14], F, rtol=1e-12, atol=1e-12 * np.abs(F).max())
+        A = P[5:]
+        ac = ora.acoustic(MAT, 1e-3, A)
+        O = out[14:20]
+        np.testing.assert_allclose(O, [ac[0, 0], ac[0, 1], ac[0, 2], ac[1, 1], ac[1, 2], ac[2, 2]],
+                                   rtol=1e-11, atol=1e-11 * np.abs(ac).max())
+        assert out[20] == pytest.approx(P[1], rel=1e-13)
+
+
+def _central(fwd, x, k, h, ob, nout):
+    xp, xm = x.copy(), x.copy()
+    xp[k] += h
+    xm[k] -= h
+    return ob @ (call(fwd, xp, nout) - call(fwd, xm, nout)) / (2 * h)
+
+
+def _check_vjp(fwd, vj, x, nout, rng, rel=1e-7, skip=()):
+    """Richardson-extrapolated central differences (error O(h^4) + eps/h with
+    h = 1e-4 |x_k|) against the reverse sweep contracted with a random cotangent."""
+    ob = rng.standard_normal(nout)
+    g = vjp(vj, x, ob)
+    for k in range(len(x)):
+        if k in skip:
+            continue
+        h = 1e-4 * max(abs(x[k]), 1e-3)
+        fd = (4 * _central(fwd, x, k, h / 2, ob, nout) - _central(fwd, x, k, h, ob, nout)) / 3
+        tol = rel * max(abs(fd), abs(g[k])) + 1e-11 * np.abs(g).max()
+        assert abs(g[k] - fd) <= tol, (k, g[k], fd)
+
+
+def test_cell_vjp_against_finite_differences(gen, ora):
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        Q, _ = rand_state(ora, rng)
+        # rho0, cv, t0 are constants of the adjoint (not differentiated)
+        _check_vjp(gen.gen_cell_fwd, gen.gen_cell_vjp, cell_in(Q), 21, rng, skip=(16, 17, 18))
+
+
+def test_prim2cons_primal_and_vjp(gen, ora):
+    rng = np.random.default_rng(2)
+    for _ in range(5):
+        Q, P = rand_state(ora, rng)
+        x = np.concatenate([P[1:4], P[5:14], [P[4]], [MAT[4], MAT[1], MAT[0], MAT[2], MAT[3]]])
+        out = call(gen.gen_p2c_fwd, x, 14)
+        np.testing.assert_allclose(out, Q, rtol=1e-12, atol=1e-14)
+        _check_vjp(gen.gen_p2c_fwd, gen.gen_p2c_vjp, x, 14, rng, skip=(15, 16, 17))
+
+
+def test_loss_vjp(gen, ora):
+    rng = np.random.default_rng(3)
+    Q, P = rand_state(ora, rng)
+    x = np.concatenate([Q, [P[0] * 1.01, P[1] + 0.1, MAT[0]]])
+    out = call(gen.gen_loss_fwd, x, 1)
+    assert out[0] == pytest.approx((0.01 * P[0]) ** 2 + 0.1 ** 2, rel=1e-6)
+    _check_vjp(gen.gen_loss_fwd, gen.gen_loss_vjp, x, 1, rng, skip=(16,))

@@ -150,18 +150,22 @@ def test_batch_per_problem_t_end(ora):
 
 def test_autotune_on_reused_memory_matches_clean_run(ora):
     """Device memory full of NaNs from an earlier allocation must never leak in:
-    the autotuner's save/restore copies are ordered with its kernels."""
-    import torch
+    the autotuner's save/restore copies are ordered with its kernels.  The
+    poisoning goes through this library's own allocations (a second HIP
+    runtime in the process, such as torch's, may not see the device once this
+    library's runtime holds it)."""
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    from hyperelasticsolver_amd.batched import Batch
     print("[batch] autotune on NaN-filled reused memory", flush=True)
     ncells = [300, 4000, 777, 5000] * 4
     b, *_ = _build(ora, ncells, seed=6)
     b.run(6, mode="eager")
     clean = b.get_state()
     b.close()
-    x = torch.full((1 << 29,), float("nan"), dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
-    del x
-    torch.cuda.empty_cache()
+    for n in (ncells, [sum(ncells) * 8]):
+        p = Batch([Barton2009()] * len(n), n)
+        p.set_state(np.full((14, sum(n)), np.nan, order="F"))
+        p.close()
     b, *_ = _build(ora, ncells, seed=6)
     b.autotune(steps=3)
     b.set_launch(256, True)
