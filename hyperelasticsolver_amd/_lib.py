@@ -58,6 +58,14 @@ _SIGS = {
                        ctypes.POINTER(ctypes.c_float), _dp],
     "hes_solver_time_steps": [_vp, _ll, ctypes.POINTER(ctypes.c_float)],
     "hes_copy_bandwidth": [_ll, _i, _dp],
+    "hes_adjoint_create": [ctypes.POINTER(_vp), _ll, _dp, _d, _d, _ll, _ll],
+    "hes_adjoint_destroy": [_vp],
+    "hes_adjoint_set_initial": [_vp, _i, _dp, _dp],
+    "hes_adjoint_set_target": [_vp, _dp, _dp],
+    "hes_adjoint_forward": [_vp, _dp, ctypes.POINTER(ctypes.c_float)],
+    "hes_adjoint_backward": [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_float)],
+    "hes_adjoint_get_state": [_vp, _dp],
+    "hes_adjoint_info": [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_ll)],
     "hes_batch_create": [ctypes.POINTER(_vp), _i, _i, _i, ctypes.POINTER(_ll), _dp, _dp, _dp],
     "hes_batch_destroy": [_vp],
     "hes_batch_info": [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_i), ctypes.POINTER(_i),

@@ -1,0 +1,996 @@
+// hes_adjoint.hip -- reverse mode of a fixed-length single-phase Rusanov run:
+// the gradient of a final-state misfit
+//     L = sum_i (rho_i - rho*_i)^2 + (u1_i - u1*_i)^2
+// with respect to the primitive initial states (u, A, S of each constant
+// piece) and the material constants b0 (shear speed) and c0 (bulk speed).
+//
+// Forward model, step s -> s+1 (main.jl:204-225 with NumFluxes.jl:25-60):
+//   lam_i  = |u1_i| + c_i                 (max |eigenvalue|, c = sqrt rho(Omega))
+//   lmax   = max_i lam_i,  dtdx = (cfl dx / lmax) / dx
+//   G_f    = 1/2 (F_i + F_i+1) - 1/2 max(lam_i, lam_i+1) (Q_i+1 - Q_i)
+//   Q_i   <- Q_i - dtdx (G_i+1/2 - G_i-1/2)      (interior cells; the end cells stay)
+//
+// Reverse step, with W_i = Qbar_i of state s+1 on interior cells (0 on the
+// end cells) and g_f = W_right(f) - W_left(f):
+//   Qbar_i += dtdx/2 (s_R g_R - s_L g_L) + [dQ of the cell VJP]
+//   Fbar_i  = dtdx/2 (g_L + g_R)
+//   sbar_f  = -dtdx/2 g_f . (Q_i+1 - Q_i)   -> lam of the larger side (ties: left cell)
+//   lambar  -> u1bar = sign(u1) lambar,  Omegabar = lambar / (2c) v v^T  (v: top eigenvector)
+//   dtdxbar = sum_i F_i . (g_L + g_R)/2 - sum_f s_f/2 (Q_i+1 - Q_i) . g_f
+//   lmaxbar = -dtdxbar dtdx / lmax          -> lam of the argmax cell (smallest index)
+// The per-cell VJP (flux, acoustic tensor, velocity of a cell as functions of
+// its conserved state and of b0, c0) is generated code (hes_adjoint_gen.hpp).
+//
+// Storage: every `every`-th state is kept (with its sound speeds); a backward
+// segment recomputes the states in between with the forward kernel (bit-identical
+// to the first pass), then runs the reverse steps over them.  The lmax term of
+// step s needs the global sum dtdxbar_s, so it is applied when step s-1 loads
+// Qbar_s: the one or two waves holding the argmax cell reduce the block
+// partials of step s and add lmaxbar_s * dlam/d(Q, b0, c0) of that cell
+// (computed once per segment).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "hes_host.hpp"
+#include "hes_step.hpp"
+#include "hes_adjoint_gen.hpp"
+#include "../../include/hes.h"
+
+using namespace hes;
+
+namespace {
+
+constexpr int ABLK = 256;  // threads per block of the tiled kernels (4 waves)
+constexpr int LBLK = 256;  // threads per block of the one-cell-per-thread kernels
+constexpr int NQ = 14;
+constexpr int NP = 13;     // differentiable primitives of a state: u(3), A(9), S
+constexpr int JL = 16;     // dlam/dQ (14), dlam/db0, dlam/dc0
+
+__device__ __forceinline__ double wave_sum(double v) {
+  // xor butterfly: every lane ends with the same bits (each level adds the
+  // same two operands on both partners)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Block sum of v over ABLK threads (thread 0 gets the result).
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double red[ABLK / 64];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = red[0];
+#pragma unroll
+  for (int k = 1; k < ABLK / 64; ++k) s += red[k];
+  return s;
+}
+
+__device__ __forceinline__ void block_max_atomic(double v, unsigned long long* slot) {
+  __shared__ double red[ABLK / 64];
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+#pragma unroll
+    for (int k = 1; k < ABLK / 64; ++k) m = fmax(m, red[k]);
+    if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// (value, index) with the larger value, ties to the smaller index.
+__device__ __forceinline__ void arg_better(double& v, long long& i, double ov, long long oi) {
+  if (ov > v || (ov == v && oi < i)) {
+    v = ov;
+    i = oi;
+  }
+}
+
+__device__ __forceinline__ void block_argmax(double v, long long idx, double* out_v,
+                                             long long* out_i) {
+  __shared__ double rv[ABLK / 64];
+  __shared__ long long ri[ABLK / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double ov = __shfl_xor(v, o);
+    long long oi = __shfl_xor(idx, o);
+    arg_better(v, idx, ov, oi);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    rv[w] = v;
+    ri[w] = idx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bv = rv[0];
+    long long bi = ri[0];
+#pragma unroll
+    for (int k = 1; k < ABLK / 64; ++k) arg_better(bv, bi, rv[k], ri[k]);
+    out_v[blockIdx.x] = bv;
+    out_i[blockIdx.x] = bi;
+  }
+}
+
+__device__ __forceinline__ double lmax_of(const unsigned long long* h, long long s) {
+  return __longlong_as_double((long long)h[s]);
+}
+
+// Sound speed c = sqrt(rho(Omega)) of a symmetric acoustic tensor (O11 O12 O13
+// O22 O23 O33) and dc/dO (off-diagonal entries counted twice, as each stands
+// for two matrix entries).  rho(Omega) is the eigenvalue of largest modulus,
+// found as spectral_radius_sym does; its eigenvector is the largest cross
+// product of two rows of Omega - mu I.
+__device__ __forceinline__ double speed_and_grad(const double O[6], double g[6]) {
+  const double a = O[0], b = O[3], c = O[5], d = O[1], e = O[2], f = O[4];
+  const double c2 = a + b + c;
+  const double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
+  const double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
+  const double r1 = fabs(d) + fabs(e), r2 = fabs(d) + fabs(f), r3 = fabs(e) + fabs(f);
+  const double lb = fmin(a - r1, fmin(b - r2, c - r3));
+  const double hd = 0.5 * (b - c);
+  const double lD = 0.5 * (b + c) + sqrt(hd * hd + f * f);
+  const double ww = d * d + e * e;
+  double ub = fmax(a, lD) + sqrt(ww);
+  if (a > lD) ub = fmin(ub, a + ww * rcp(a - lD));
+  double mu = cubic_top_root(c2, c1, c0, ub, lb);
+  double sgn = mu < 0.0 ? -1.0 : 1.0;
+  double rad = fabs(mu);
+  if (lb < -rad) {
+    double bot = cubic_bottom_root(c2, c1, c0, lb, ub);
+    if (fabs(bot) > rad) {
+      mu = bot;
+      rad = fabs(bot);
+      sgn = bot < 0.0 ? -1.0 : 1.0;
+    }
+  }
+  const double A = a - mu, B = b - mu, C = c - mu;
+  // rows (A d e), (d B f), (e f C)
+  double x1 = d * f - e * B, y1 = e * d - A * f, z1 = A * B - d * d;
+  double x2 = d * C - e * f, y2 = e * e - A * C, z2 = A * f - d * e;
+  double x3 = B * C - f * f, y3 = f * e - d * C, z3 = d * f - B * e;
+  double n1 = x1 * x1 + y1 * y1 + z1 * z1;
+  double n2 = x2 * x2 + y2 * y2 + z2 * z2;
+  double n3 = x3 * x3 + y3 * y3 + z3 * z3;
+  double vx = x1, vy = y1, vz = z1, nn = n1;
+  if (n2 > nn) { vx = x2; vy = y2; vz = z2; nn = n2; }
+  if (n3 > nn) { vx = x3; vy = y3; vz = z3; nn = n3; }
+  const double cs = sqrt(rad);
+  // dc/dmu = sgn / (2c), dmu/dO = v v^T / |v|^2
+  const double k = nn > 0.0 ? sgn * 0.5 * rcp(cs) * rcp(nn) : 0.0;
+  g[0] = k * vx * vx;
+  g[1] = 2.0 * k * vx * vy;
+  g[2] = 2.0 * k * vx * vz;
+  g[3] = k * vy * vy;
+  g[4] = 2.0 * k * vy * vz;
+  g[5] = k * vz * vz;
+  return cs;
+}
+
+// ---------------------------------------------------------------------------
+// Forward kernels
+// ---------------------------------------------------------------------------
+
+// Conserved states of the constant pieces from their primitives.
+__global__ void k_adj_p2c(const double* __restrict__ prims, const double* __restrict__ gin,
+                          double* __restrict__ states, int nst) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nst) return;
+  double in[hes_gen::P2C_IN];
+#pragma unroll
+  for (int r = 0; r < NP; ++r) in[r] = prims[NP * k + r];
+#pragma unroll
+  for (int r = 0; r < 5; ++r) in[NP + r] = gin[r];
+  double out[NQ];
+  hes_gen::prim2cons_fwd(in, out);
+#pragma unroll
+  for (int r = 0; r < NQ; ++r) states[NQ * k + r] = out[r];
+}
+
+// Piecewise-constant fill: cells [bounds[k], bounds[k+1]) take state k.
+__global__ void k_adj_fill(const double* __restrict__ states, const long long* __restrict__ bounds,
+                           int nst, double* __restrict__ Q, long long n, long long pitch) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int k = 0;
+  while (k + 1 < nst && i >= bounds[k + 1]) ++k;
+#pragma unroll
+  for (int r = 0; r < NQ; ++r) Q[r * pitch + i] = states[NQ * k + r];
+}
+
+// The end cells never change: copy them from src into every state slot.
+__global__ void k_adj_ends(const double* __restrict__ src, double* const* __restrict__ dst,
+                           int ndst, long long n, long long pitch) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ndst * 2 * NQ) return;
+  int slot = t / (2 * NQ), rem = t % (2 * NQ);
+  int r = rem >> 1;
+  long long i = (rem & 1) ? n - 1 : 0;
+  dst[slot][r * pitch + i] = src[r * pitch + i];
+}
+
+__global__ void k_adj_zero(double* __restrict__ p, long long m) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) p[t] = 0.0;
+}
+
+// Sound speeds of state 0 and its max CFL speed (main.jl:204-212).
+__global__ __launch_bounds__(LBLK) void k_adj_speeds(const double* __restrict__ Q,
+                                                     double* __restrict__ C, long long n,
+                                                     long long pitch, MatDev m,
+                                                     unsigned long long* lmax0, int* status) {
+  long long i = (long long)blockIdx.x * LBLK + threadIdx.x;
+  double lam = 0.0;
+  int bad = 0;
+  if (i < n) {
+    double q[NQ], cc[1];
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) q[r] = Q[r * pitch + i];
+    lam = cell_speeds<1, 1>(&m, q, cc, bad);
+    C[i] = cc[0];
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1);
+  block_max_atomic(lam, lmax0);
+}
+
+struct AdjFwdArgs {
+  const double* __restrict__ Qin;
+  double* __restrict__ Qout;
+  const double* __restrict__ Cin;
+  double* __restrict__ Cout;
+  unsigned long long* lmax;  // lmax[s] = max CFL speed of state s (double bits)
+  double* blk_val;           // optional: per-block argmax of the CFL speed of state s
+  long long* blk_idx;
+  int* status;
+  long long n, pitch, s;
+  double cfl, dx;
+  int write_lmax;            // max-reduce the new state's speeds into lmax[s + 1]
+  MatDev mat;
+};
+
+// One time step: the solver's chunk step (so states match a plain run bit for
+// bit) plus the bookkeeping of the reverse pass.
+__global__ __launch_bounds__(ABLK) __attribute__((amdgpu_waves_per_eu(3))) void k_adj_fwd(
+    AdjFwdArgs a) {
+  const int lane = threadIdx.x & 63;
+  const long long wave = (xcd_block() * ABLK + threadIdx.x) >> 6;
+  const long long i = wave * TILE + lane - 1;
+  const ChunkCtx ctx = {&a.mat, a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, 0, a.n, a.cfl, a.dx};
+  double q[NQ], cc[1];
+  load_chunk<1>(ctx, i, q, cc);
+  const double lmax = lmax_of(a.lmax, a.s);
+  double lam_cur;
+  const double lam = chunk_step<1, FLUX_RUSANOV, 1>(ctx, i, q, cc, lmax, &lam_cur);
+  if (a.write_lmax) block_max_atomic(lam, a.lmax + a.s + 1);
+  if (a.blk_val) {
+    const bool owned = i >= 0 && i < a.n && lane >= 1 && lane <= TILE;
+    block_argmax(owned ? lam_cur : -1.0, owned ? i : (long long)0x7fffffffffffffffLL,
+                 a.blk_val + a.s * (long long)gridDim.x, a.blk_idx + a.s * (long long)gridDim.x);
+  }
+}
+
+// Argmax cell of the CFL speed of every state from the per-block records.
+__global__ __launch_bounds__(64) void k_adj_argmax(const double* __restrict__ blk_val,
+                                                   const long long* __restrict__ blk_idx, int nb,
+                                                   long long* __restrict__ istar) {
+  const long long s = blockIdx.x;
+  const int lane = threadIdx.x;
+  double v = -2.0;
+  long long i = 0x7fffffffffffffffLL;
+  for (int b = lane; b < nb; b += 64) arg_better(v, i, blk_val[s * nb + b], blk_idx[s * nb + b]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double ov = __shfl_xor(v, o);
+    long long oi = __shfl_xor(i, o);
+    arg_better(v, i, ov, oi);
+  }
+  if (lane == 0) istar[s] = i;
+}
+
+// Misfit of the final state and its gradient Qbar_n (HyperelasticityMPh.jl:106-121
+// gives rho and u1 of a cell).
+__global__ __launch_bounds__(LBLK) void k_adj_loss(const double* __restrict__ Q,
+                                                   const double* __restrict__ rho_t,
+                                                   const double* __restrict__ u_t, double rho0,
+                                                   double* __restrict__ Qb, double* __restrict__ part,
+                                                   long long n, long long pitch) {
+  long long i = (long long)blockIdx.x * LBLK + threadIdx.x;
+  double l = 0.0;
+  if (i < n) {
+    double in[hes_gen::LOSS_IN];
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) in[r] = Q[r * pitch + i];
+    in[NQ] = rho_t[i];
+    in[NQ + 1] = u_t[i];
+    in[NQ + 2] = rho0;
+    double out[1];
+    hes_gen::loss_fwd(in, out);
+    l = out[0];
+    double ob[1] = {1.0}, ib[hes_gen::LOSS_IN];
+#pragma unroll
+    for (int r = 0; r < hes_gen::LOSS_IN; ++r) ib[r] = 0.0;
+    hes_gen::loss_vjp(in, ob, ib);
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) Qb[r * pitch + i] = ib[r];
+  }
+  double s = block_sum(l);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Reverse kernels
+// ---------------------------------------------------------------------------
+
+// dlam/d(Q, b0, c0) of the argmax cell of each state s0 .. s0+cnt-1 (lane j
+// handles state s0 + j; its state lives in Q0 for j = 0, else in
+// seg + (j - 1) * stride).
+__global__ __launch_bounds__(64) void k_adj_lam(const double* __restrict__ Q0,
+                                                const double* __restrict__ seg, long long stride,
+                                                long long pitch, const long long* __restrict__ istar,
+                                                const double* __restrict__ gin, long long s0,
+                                                int cnt, long long n, double* __restrict__ Jlam) {
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  if (j >= cnt) return;
+  const long long s = s0 + j;
+  const long long i = istar[s];
+  if (i < 0 || i >= n) {  // no finite CFL speed was recorded: no lmax term
+#pragma unroll
+    for (int r = 0; r < JL; ++r) Jlam[s * JL + r] = 0.0;
+    return;
+  }
+  const double* Q = j == 0 ? Q0 : seg + (long long)(j - 1) * stride;
+  double in[hes_gen::CELL_IN];
+#pragma unroll
+  for (int r = 0; r < NQ; ++r) in[r] = Q[r * pitch + i];
+#pragma unroll
+  for (int r = 0; r < 5; ++r) in[NQ + r] = gin[r];
+  double out[hes_gen::CELL_OUT];
+  hes_gen::cell_fwd(in, out);
+  double ob[hes_gen::CELL_OUT], ib[hes_gen::CELL_IN];
+#pragma unroll
+  for (int r = 0; r < NQ; ++r) ob[r] = 0.0;
+  speed_and_grad(out + NQ, ob + NQ);
+  ob[20] = out[20] < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+  for (int r = 0; r < hes_gen::CELL_IN; ++r) ib[r] = 0.0;
+  hes_gen::cell_vjp(in, ob, ib);
+#pragma unroll
+  for (int r = 0; r < JL; ++r) Jlam[s * JL + r] = ib[r];
+}
+
+struct AdjStepArgs {
+  const double* __restrict__ Q;      // state s
+  const double* __restrict__ Qb_in;  // Qbar of state s+1 (without its lmax term)
+  double* __restrict__ Qb_out;       // Qbar of state s (without its lmax term)
+  const unsigned long long* lmax;
+  const long long* istar;
+  const double* Jlam;
+  const double* dtp_next;  // block partials of dtdxbar of step s+1
+  double* dtp;             // block partials of dtdxbar of step s
+  double* thb;             // running (b0bar, c0bar) per block
+  long long n, pitch, s;
+  double cfl, dx;
+  int nb, has_next;
+  double gin[5];           // b0, c0, rho0, cv, t0
+};
+
+__global__ __launch_bounds__(ABLK) void k_adj_step(AdjStepArgs a) {
+  const int lane = threadIdx.x & 63;
+  const long long wave = (xcd_block() * ABLK + threadIdx.x) >> 6;
+  const long long i = wave * TILE + lane - 1;
+  const bool inr = i >= 0 && i < a.n;
+  const bool owned = inr && lane >= 1 && lane <= TILE;
+  const long long ic = i < 0 ? 0 : (i >= a.n ? a.n - 1 : i);
+  const long long pitch = a.pitch;
+  double q[NQ], qb[NQ];
+#pragma unroll
+  for (int r = 0; r < NQ; ++r) {
+    q[r] = ld_stream(a.Q + r * pitch + ic);
+    qb[r] = inr ? ld_stream(a.Qb_in + r * pitch + ic) : 0.0;
+  }
+  double th0 = 0.0, th1 = 0.0;
+  // lmax term of step s+1 at its argmax cell
+  if (a.has_next) {
+    const long long is1 = a.istar[a.s + 1];
+    if (__any(inr && i == is1)) {
+      double acc = 0.0;
+      for (int b = lane; b < a.nb; b += 64) acc += a.dtp_next[b];
+      acc = wave_sum(acc);
+      const double lm1 = lmax_of(a.lmax, a.s + 1);
+      const double dtdx1 = (a.cfl * a.dx / lm1) / a.dx;
+      const double lbar = -acc * dtdx1 / lm1;
+      if (inr && i == is1) {
+        const double* J = a.Jlam + (a.s + 1) * JL;
+#pragma unroll
+        for (int r = 0; r < NQ; ++r) qb[r] += lbar * J[r];
+        if (owned) {
+          th0 += lbar * J[NQ];
+          th1 += lbar * J[NQ + 1];
+        }
+      }
+    }
+  }
+  const bool interior = inr && i > 0 && i < a.n - 1;
+
+  // primal of the cell: flux, acoustic tensor, u1; its CFL speed
+  double in[hes_gen::CELL_IN];
+#pragma unroll
+  for (int r = 0; r < NQ; ++r) in[r] = q[r];
+#pragma unroll
+  for (int r = 0; r < 5; ++r) in[NQ + r] = a.gin[r];
+  double out[hes_gen::CELL_OUT];
+  hes_gen::cell_fwd(in, out);
+  double ob[hes_gen::CELL_OUT];
+  const double c = speed_and_grad(out + NQ, ob + NQ);
+  const double u1 = out[20];
+  const double lam = fabs(u1) + c;
+  const double lam_r = __shfl_down(lam, 1), lam_l = __shfl_up(lam, 1);
+  const double sR = fmax(lam, lam_r), sL = fmax(lam_l, lam);
+  const double dtdx = (a.cfl * a.dx / lmax_of(a.lmax, a.s)) / a.dx;
+  const double hdt = 0.5 * dtdx;
+
+  double sdR = 0.0, sdL = 0.0, fdot = 0.0;
+#pragma unroll
+  for (int r = 0; r < NQ; ++r) {
+    const double w = interior ? qb[r] : 0.0;
+    const double wr = __shfl_down(w, 1), wl = __shfl_up(w, 1);
+    const double qr = __shfl_down(q[r], 1), ql = __shfl_up(q[r], 1);
+    const double gR = wr - w, gL = w - wl;
+    const double gs = gL + gR;
+    ob[r] = hdt * gs;
+    fdot += out[r] * gs;
+    sdR += gR * (qr - q[r]);
+    sdL += gL * (q[r] - ql);
+    qb[r] += hdt * (sR * gR - sL * gL);
+  }
+  const double lbar = (lam >= lam_r ? -hdt * sdR : 0.0) + (lam > lam_l ? -hdt * sdL : 0.0);
+#pragma unroll
+  for (int r = NQ; r < NQ + 6; ++r) ob[r] *= lbar;
+  ob[20] = u1 < 0.0 ? -lbar : lbar;
+  const double dts = owned ? 0.5 * fdot - 0.5 * sR * sdR : 0.0;
+
+  double ib[hes_gen::CELL_IN];
+#pragma unroll
+  for (int r = 0; r < hes_gen::CELL_IN; ++r) ib[r] = 0.0;
+  hes_gen::cell_vjp(in, ob, ib);
+  if (owned) {
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) a.Qb_out[r * pitch + i] = qb[r] + ib[r];
+    th0 += ib[NQ];
+    th1 += ib[NQ + 1];
+  }
+  const double bd = block_sum(dts);
+  const double b0s = block_sum(th0);
+  const double c0s = block_sum(th1);
+  if (threadIdx.x == 0) {
+    a.dtp[blockIdx.x] = bd;
+    a.thb[2 * blockIdx.x] += b0s;
+    a.thb[2 * blockIdx.x + 1] += c0s;
+  }
+}
+
+// Per-block, per-piece sums of Qbar of state 0 (pieces are contiguous cell
+// ranges, so a block meets only a few of them).
+__global__ __launch_bounds__(LBLK) void k_adj_q0sum(const double* __restrict__ Qb,
+                                                    const long long* __restrict__ bounds, int nst,
+                                                    double* __restrict__ part, long long n,
+                                                    long long pitch) {
+  const long long b0 = (long long)blockIdx.x * LBLK;
+  const long long i = b0 + threadIdx.x;
+  const long long last = (b0 + LBLK < n ? b0 + LBLK : n) - 1;
+  int klo = 0;
+  while (klo + 1 < nst && b0 >= bounds[klo + 1]) ++klo;
+  int khi = klo;
+  while (khi + 1 < nst && last >= bounds[khi + 1]) ++khi;
+  int k = 0;
+  while (k + 1 < nst && i >= bounds[k + 1]) ++k;
+  double* out = part + (long long)blockIdx.x * nst * NQ;
+  if (threadIdx.x == 0)
+    for (int kk = 0; kk < nst; ++kk)
+      if (kk < klo || kk > khi)
+        for (int r = 0; r < NQ; ++r) out[kk * NQ + r] = 0.0;
+  for (int kk = klo; kk <= khi; ++kk) {
+    for (int r = 0; r < NQ; ++r) {
+      double v = (i < n && k == kk) ? Qb[r * pitch + i] : 0.0;
+      double s = block_sum(v);
+      if (threadIdx.x == 0) out[kk * NQ + r] = s;
+    }
+  }
+}
+
+// Final reduction: Qbar per piece (with the lmax term of step 0), the VJP of
+// prim2cons per piece, and the material gradient.  One wave.
+__global__ __launch_bounds__(64) void k_adj_final(
+    const double* __restrict__ part, int nbl, const double* __restrict__ thb, int nb,
+    const double* __restrict__ dtp0, const unsigned long long* __restrict__ lmax,
+    const long long* __restrict__ istar, const double* __restrict__ Jlam,
+    const long long* __restrict__ bounds, int nst, const double* __restrict__ prims,
+    const double* __restrict__ gin, double cfl, double dx, double* __restrict__ sums,
+    double* __restrict__ gout) {
+  const int lane = threadIdx.x;
+  // lmax term of step 0
+  double acc = 0.0;
+  for (int b = lane; b < nb; b += 64) acc += dtp0[b];
+  acc = wave_sum(acc);
+  const double lm0 = lmax_of(lmax, 0);
+  const double dtdx0 = (cfl * dx / lm0) / dx;
+  const double lbar = -acc * dtdx0 / lm0;
+  const long long i0 = istar[0];
+  int k0 = 0;
+  while (k0 + 1 < nst && i0 >= bounds[k0 + 1]) ++k0;
+  for (int kr = 0; kr < nst * NQ; ++kr) {
+    double v = 0.0;
+    for (int b = lane; b < nbl; b += 64) v += part[(long long)b * nst * NQ + kr];
+    v = wave_sum(v);
+    const int k = kr / NQ, r = kr % NQ;
+    if (k == k0) v += lbar * Jlam[r];
+    if (lane == 0) sums[kr] = v;
+  }
+  double tb0 = 0.0, tc0 = 0.0;
+  for (int b = lane; b < nb; b += 64) {
+    tb0 += thb[2 * b];
+    tc0 += thb[2 * b + 1];
+  }
+  tb0 = wave_sum(tb0);
+  tc0 = wave_sum(tc0);
+  __syncthreads();
+  double pb0 = 0.0, pc0 = 0.0;
+  for (int k = lane; k < nst; k += 64) {
+    double in[hes_gen::P2C_IN], ib[hes_gen::P2C_IN];
+#pragma unroll
+    for (int r = 0; r < NP; ++r) in[r] = prims[NP * k + r];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) in[NP + r] = gin[r];
+#pragma unroll
+    for (int r = 0; r < hes_gen::P2C_IN; ++r) ib[r] = 0.0;
+    hes_gen::prim2cons_vjp(in, sums + NQ * k, ib);
+#pragma unroll
+    for (int r = 0; r < NP; ++r) gout[NP * k + r] = ib[r];
+    pb0 += ib[NP];
+    pc0 += ib[NP + 1];
+  }
+  pb0 = wave_sum(pb0);
+  pc0 = wave_sum(pc0);
+  if (lane == 0) {
+    gout[NP * nst] = tb0 + pb0 + lbar * Jlam[NQ];
+    gout[NP * nst + 1] = tc0 + pc0 + lbar * Jlam[NQ + 1];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Handle
+// ---------------------------------------------------------------------------
+struct hes_adjoint {
+  long long n, pitch, nsteps, every, nck;
+  int nb;    // blocks of the tiled kernels
+  int nbl;   // blocks of the one-cell-per-thread kernels
+  double cfl, dx;
+  double mat8[8];
+  MatDev mat;
+  int nst;
+  double* ck = nullptr;    // nck slots of 15 x pitch (Q, then C)
+  double* seg = nullptr;   // every - 1 slots of 14 x pitch
+  double* fq[2] = {nullptr, nullptr};
+  double* fc[2] = {nullptr, nullptr};
+  double* qb[2] = {nullptr, nullptr};
+  unsigned long long* lmax = nullptr;  // nsteps + 1
+  double* blk_val = nullptr;           // nsteps x nb
+  long long* blk_idx = nullptr;
+  long long* istar = nullptr;          // nsteps
+  double* Jlam = nullptr;              // nsteps x 16
+  double* dtp = nullptr;               // 2 x nb
+  double* thb = nullptr;               // 2 x nb
+  double* lossp = nullptr;             // nbl
+  double* part = nullptr;              // nbl x nst x 14
+  double* sums = nullptr;              // nst x 14
+  double* gout = nullptr;              // 13 nst + 2
+  double* gin = nullptr;               // b0, c0, rho0, cv, t0
+  double* prims = nullptr;             // 13 x nst
+  double* states = nullptr;            // 14 x nst
+  long long* bounds = nullptr;         // nst + 1
+  double* rho_t = nullptr;
+  double* u_t = nullptr;
+  double** slots = nullptr;            // device list of state slots (end-cell copies)
+  int nslots = 0;
+  int* status = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t gf = nullptr, gb = nullptr;
+  bool has_init = false, has_target = false, fwd_done = false;
+};
+
+namespace {
+
+double* ck_q(hes_adjoint* h, long long c) { return h->ck + c * 15 * h->pitch; }
+double* ck_c(hes_adjoint* h, long long c) { return ck_q(h, c) + 14 * h->pitch; }
+
+// State slot of state s in the first forward pass.
+void fwd_slot(hes_adjoint* h, long long s, double** q, double** c) {
+  if (s % h->every == 0 && s < h->nsteps) {
+    *q = ck_q(h, s / h->every);
+    *c = ck_c(h, s / h->every);
+  } else {
+    *q = h->fq[s & 1];
+    *c = h->fc[s & 1];
+  }
+}
+
+// State slot of state s inside its backward segment (s0 = first state).
+double* seg_slot(hes_adjoint* h, long long s, long long s0) {
+  if (s == s0) return ck_q(h, s0 / h->every);
+  return h->seg + (s - s0 - 1) * 14 * h->pitch;
+}
+
+AdjFwdArgs fwd_args(hes_adjoint* h, long long s, const double* qi, double* qo, const double* ci,
+                    double* co, bool first_pass) {
+  AdjFwdArgs a;
+  a.Qin = qi;
+  a.Qout = qo;
+  a.Cin = ci;
+  a.Cout = co;
+  a.lmax = h->lmax;
+  a.blk_val = first_pass ? h->blk_val : nullptr;
+  a.blk_idx = first_pass ? h->blk_idx : nullptr;
+  a.status = h->status;
+  a.n = h->n;
+  a.pitch = h->pitch;
+  a.s = s;
+  a.cfl = h->cfl;
+  a.dx = h->dx;
+  a.write_lmax = first_pass ? 1 : 0;
+  a.mat = h->mat;
+  return a;
+}
+
+hipError_t record_forward(hes_adjoint* h) {
+  hipStream_t st = h->stream;
+  const long long n = h->n;
+  const unsigned nbl = (unsigned)h->nbl;
+  hipLaunchKernelGGL(k_adj_p2c, dim3((h->nst + 63) / 64), dim3(64), 0, st, h->prims, h->gin,
+                     h->states, h->nst);
+  hipLaunchKernelGGL(k_adj_fill, dim3(nbl), dim3(LBLK), 0, st, h->states, h->bounds, h->nst,
+                     ck_q(h, 0), n, h->pitch);
+  hipLaunchKernelGGL(k_adj_ends, dim3((h->nslots * 2 * NQ + 63) / 64), dim3(64), 0, st,
+                     ck_q(h, 0), h->slots, h->nslots, n, h->pitch);
+  hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((h->nsteps + 1 + 255) / 256)), dim3(256), 0, st,
+                     (double*)h->lmax, h->nsteps + 1);
+  hipLaunchKernelGGL(k_adj_speeds, dim3(nbl), dim3(LBLK), 0, st, ck_q(h, 0), ck_c(h, 0), n,
+                     h->pitch, h->mat, h->lmax, h->status);
+  for (long long s = 0; s < h->nsteps; ++s) {
+    double *qi, *ci, *qo, *co;
+    fwd_slot(h, s, &qi, &ci);
+    fwd_slot(h, s + 1, &qo, &co);
+    hipLaunchKernelGGL(k_adj_fwd, dim3((unsigned)h->nb), dim3(ABLK), 0, st,
+                       fwd_args(h, s, qi, qo, ci, co, true));
+  }
+  hipLaunchKernelGGL(k_adj_argmax, dim3((unsigned)h->nsteps), dim3(64), 0, st, h->blk_val,
+                     h->blk_idx, h->nb, h->istar);
+  double *qn, *cn;
+  fwd_slot(h, h->nsteps, &qn, &cn);
+  hipLaunchKernelGGL(k_adj_loss, dim3(nbl), dim3(LBLK), 0, st, qn, h->rho_t, h->u_t, h->mat8[0],
+                     h->qb[h->nsteps & 1], h->lossp, n, h->pitch);
+  return hipGetLastError();
+}
+
+hipError_t record_backward(hes_adjoint* h) {
+  hipStream_t st = h->stream;
+  hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((2 * h->nb + 255) / 256)), dim3(256), 0, st,
+                     h->thb, (long long)2 * h->nb);
+  for (long long c = h->nck - 1; c >= 0; --c) {
+    const long long s0 = c * h->every;
+    const long long end = std::min(s0 + h->every, h->nsteps);
+    // recompute states s0+1 .. end-1
+    for (long long s = s0; s + 1 < end; ++s) {
+      const double* ci = s == s0 ? ck_c(h, c) : h->fc[(s - s0) & 1];
+      double* co = h->fc[(s - s0 + 1) & 1];
+      hipLaunchKernelGGL(k_adj_fwd, dim3((unsigned)h->nb), dim3(ABLK), 0, st,
+                         fwd_args(h, s, seg_slot(h, s, s0), seg_slot(h, s + 1, s0), ci, co, false));
+    }
+    const int cnt = (int)(end - s0);
+    hipLaunchKernelGGL(k_adj_lam, dim3((cnt + 63) / 64), dim3(64), 0, st, ck_q(h, c), h->seg,
+                       14 * h->pitch, h->pitch, h->istar, h->gin, s0, cnt, h->n, h->Jlam);
+    for (long long s = end - 1; s >= s0; --s) {
+      AdjStepArgs a;
+      a.Q = seg_slot(h, s, s0);
+      a.Qb_in = h->qb[(s + 1) & 1];
+      a.Qb_out = h->qb[s & 1];
+      a.lmax = h->lmax;
+      a.istar = h->istar;
+      a.Jlam = h->Jlam;
+      a.dtp_next = h->dtp + ((s + 1) & 1) * h->nb;
+      a.dtp = h->dtp + (s & 1) * h->nb;
+      a.thb = h->thb;
+      a.n = h->n;
+      a.pitch = h->pitch;
+      a.s = s;
+      a.cfl = h->cfl;
+      a.dx = h->dx;
+      a.nb = h->nb;
+      a.has_next = s + 1 < h->nsteps ? 1 : 0;
+      a.gin[0] = h->mat8[4];
+      a.gin[1] = h->mat8[1];
+      a.gin[2] = h->mat8[0];
+      a.gin[3] = h->mat8[2];
+      a.gin[4] = h->mat8[3];
+      hipLaunchKernelGGL(k_adj_step, dim3((unsigned)h->nb), dim3(ABLK), 0, st, a);
+    }
+  }
+  hipLaunchKernelGGL(k_adj_q0sum, dim3((unsigned)h->nbl), dim3(LBLK), 0, st, h->qb[0], h->bounds,
+                     h->nst, h->part, h->n, h->pitch);
+  hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(64), 0, st, h->part, h->nbl, h->thb, h->nb,
+                     h->dtp, h->lmax, h->istar, h->Jlam, h->bounds, h->nst, h->prims, h->gin,
+                     h->cfl, h->dx, h->sums, h->gout);
+  return hipGetLastError();
+}
+
+hipError_t capture(hes_adjoint* h, bool fwd, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  hipError_t er = fwd ? record_forward(h) : record_backward(h);
+  e = hipStreamEndCapture(h->stream, &g);
+  if (er != hipSuccess) e = er;
+  if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  return e;
+}
+
+template <class T>
+hipError_t dalloc(T** p, size_t count) {
+  return hipMalloc((void**)p, sizeof(T) * (count > 0 ? count : 1));
+}
+
+}  // namespace
+
+extern "C" {
+
+int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* material, double cfl,
+                       double dx, long long nsteps, long long every) {
+  if (!out) return fail(HES_ERR_ARG, "null handle pointer");
+  *out = nullptr;
+  if (ncells < 3) return fail(HES_ERR_ARG, "need at least 3 cells");
+  if (nsteps < 1) return fail(HES_ERR_ARG, "need at least one step");
+  if (every < 1) return fail(HES_ERR_ARG, "checkpoint interval must be positive");
+  if (!material || !mat_ok(material)) return fail(HES_ERR_ARG, "invalid material constants");
+  if (!(material[5] == 1.0 && material[6] == 3.0 && material[7] == 2.0))
+    return fail(HES_ERR_ARG, "the adjoint supports the exponents alpha=1, beta=3, gamma=2");
+  if (!(cfl > 0) || !(dx > 0)) return fail(HES_ERR_ARG, "cfl and dx must be positive");
+  if (nsteps > 0x7fffffffLL) return fail(HES_ERR_ARG, "too many steps");
+  hes_adjoint* h = new hes_adjoint();
+  h->n = ncells;
+  h->pitch = (ncells + 63) / 64 * 64;
+  h->nsteps = nsteps;
+  h->every = every < nsteps ? every : nsteps;
+  h->nck = (nsteps + h->every - 1) / h->every;
+  const long long nwaves = (ncells + TILE - 1) / TILE;
+  h->nb = (int)((nwaves + ABLK / 64 - 1) / (ABLK / 64));
+  h->nbl = (int)((ncells + LBLK - 1) / LBLK);
+  h->cfl = cfl;
+  h->dx = dx;
+  std::memcpy(h->mat8, material, sizeof(h->mat8));
+  h->mat = make_mat(material);
+  h->nst = 0;
+  const size_t P = (size_t)h->pitch;
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e == hipSuccess) e = dalloc(&h->ck, (size_t)h->nck * 15 * P);
+  if (e == hipSuccess && h->every > 1) e = dalloc(&h->seg, (size_t)(h->every - 1) * 14 * P);
+  for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+    e = dalloc(&h->fq[b], 14 * P);
+    if (e == hipSuccess) e = dalloc(&h->fc[b], P);
+    if (e == hipSuccess) e = dalloc(&h->qb[b], 14 * P);
+  }
+  if (e == hipSuccess) e = dalloc(&h->lmax, (size_t)nsteps + 1);
+  if (e == hipSuccess) e = dalloc(&h->blk_val, (size_t)nsteps * h->nb);
+  if (e == hipSuccess) e = dalloc(&h->blk_idx, (size_t)nsteps * h->nb);
+  if (e == hipSuccess) e = dalloc(&h->istar, (size_t)nsteps);
+  if (e == hipSuccess) e = dalloc(&h->Jlam, (size_t)nsteps * JL);
+  if (e == hipSuccess) e = dalloc(&h->dtp, (size_t)2 * h->nb);
+  if (e == hipSuccess) e = dalloc(&h->thb, (size_t)2 * h->nb);
+  if (e == hipSuccess) e = dalloc(&h->lossp, (size_t)h->nbl);
+  if (e == hipSuccess) e = dalloc(&h->gin, 5);
+  if (e == hipSuccess) e = dalloc(&h->rho_t, P);
+  if (e == hipSuccess) e = dalloc(&h->u_t, P);
+  if (e == hipSuccess) e = dalloc(&h->status, 4);
+  if (e == hipSuccess) e = zero_on(h->stream, h->status, 4 * sizeof(int));
+  if (e == hipSuccess) {
+    // gin = (b0, c0, rho0, cv, t0): the inputs of the generated routines
+    const double g[5] = {material[4], material[1], material[0], material[2], material[3]};
+    e = copy_on(h->stream, h->gin, g, sizeof(g), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) {
+    // every state slot whose end cells must be filled
+    std::vector<double*> sl;
+    for (long long c = 1; c < h->nck; ++c) sl.push_back(ck_q(h, c));
+    for (long long j = 0; j + 1 < h->every; ++j) sl.push_back(h->seg + j * 14 * P);
+    sl.push_back(h->fq[0]);
+    sl.push_back(h->fq[1]);
+    h->nslots = (int)sl.size();
+    e = dalloc(&h->slots, sl.size());
+    if (e == hipSuccess)
+      e = copy_on(h->stream, h->slots, sl.data(), sizeof(double*) * sl.size(),
+                  hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    g_err = std::string("adjoint allocation failed: ") + hipGetErrorString(e);
+    hes_adjoint_destroy(h);
+    return HES_ERR_HIP;
+  }
+  *out = h;
+  return HES_OK;
+}
+
+int hes_adjoint_destroy(hes_adjoint* h) {
+  if (!h) return HES_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->gf) (void)hipGraphExecDestroy(h->gf);
+  if (h->gb) (void)hipGraphExecDestroy(h->gb);
+  void* bufs[] = {h->ck, h->seg, h->fq[0], h->fq[1], h->fc[0], h->fc[1], h->qb[0], h->qb[1],
+                  h->lmax, h->blk_val, h->blk_idx, h->istar, h->Jlam, h->dtp, h->thb, h->lossp,
+                  h->part, h->sums, h->gout, h->gin, h->prims, h->states, h->bounds, h->rho_t,
+                  h->u_t, h->slots, h->status};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return HES_OK;
+}
+
+int hes_adjoint_set_initial(hes_adjoint* h, int nstates, const double* prims,
+                            const double* interfaces) {
+  if (!h || !prims || nstates < 1 || (nstates > 1 && !interfaces))
+    return fail(HES_ERR_ARG, "null argument");
+  for (int j = 0; j + 1 < nstates - 1; ++j)
+    if (!(interfaces[j] <= interfaces[j + 1])) return fail(HES_ERR_ARG, "interfaces not sorted");
+  for (int k = 0; k < NP * nstates; ++k)
+    if (!std::isfinite(prims[k])) return fail(HES_ERR_ARG, "non-finite primitive state");
+  // cell i (centre x = (i + 1/2) / n) takes piece #{j : x > s_j}, as the solver's fill
+  std::vector<long long> bounds(nstates + 1);
+  bounds[0] = 0;
+  bounds[nstates] = h->n;
+  for (int k = 1; k < nstates; ++k) {
+    long long lo = 0, hi = h->n;  // first i with x_i > s_{k-1}
+    while (lo < hi) {
+      long long mid = (lo + hi) / 2;
+      double x = ((double)mid + 0.5) / (double)h->n;
+      if (x > interfaces[k - 1]) hi = mid;
+      else lo = mid + 1;
+    }
+    bounds[k] = lo;
+  }
+  if (nstates != h->nst) {
+    (void)hipStreamSynchronize(h->stream);
+    if (h->gf) { (void)hipGraphExecDestroy(h->gf); h->gf = nullptr; }
+    if (h->gb) { (void)hipGraphExecDestroy(h->gb); h->gb = nullptr; }
+    void* old[] = {h->prims, h->states, h->bounds, h->part, h->sums, h->gout};
+    for (void* p : old)
+      if (p) (void)hipFree(p);
+    h->prims = h->states = h->part = h->sums = h->gout = nullptr;
+    h->bounds = nullptr;
+    h->nst = 0;
+    hipError_t e = dalloc(&h->prims, (size_t)NP * nstates);
+    if (e == hipSuccess) e = dalloc(&h->states, (size_t)NQ * nstates);
+    if (e == hipSuccess) e = dalloc(&h->bounds, (size_t)nstates + 1);
+    if (e == hipSuccess) e = dalloc(&h->part, (size_t)h->nbl * nstates * NQ);
+    if (e == hipSuccess) e = dalloc(&h->sums, (size_t)nstates * NQ);
+    if (e == hipSuccess) e = dalloc(&h->gout, (size_t)NP * nstates + 2);
+    if (e != hipSuccess) {
+      g_err = std::string("adjoint allocation failed: ") + hipGetErrorString(e);
+      return HES_ERR_HIP;
+    }
+    h->nst = nstates;
+  }
+  HIP_TRY(copy_on(h->stream, h->prims, prims, sizeof(double) * NP * nstates,
+                  hipMemcpyHostToDevice));
+  HIP_TRY(copy_on(h->stream, h->bounds, bounds.data(), sizeof(long long) * (nstates + 1),
+                  hipMemcpyHostToDevice));
+  h->has_init = true;
+  h->fwd_done = false;
+  return HES_OK;
+}
+
+int hes_adjoint_set_target(hes_adjoint* h, const double* rho, const double* u1) {
+  if (!h || !rho || !u1) return fail(HES_ERR_ARG, "null argument");
+  HIP_TRY(copy_on(h->stream, h->rho_t, rho, sizeof(double) * h->n, hipMemcpyHostToDevice));
+  HIP_TRY(copy_on(h->stream, h->u_t, u1, sizeof(double) * h->n, hipMemcpyHostToDevice));
+  h->has_target = true;
+  h->fwd_done = false;
+  return HES_OK;
+}
+
+int hes_adjoint_forward(hes_adjoint* h, double* loss, float* elapsed_ms) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  if (!h->has_init || !h->has_target) return fail(HES_ERR_ARG, "initial state and target must be set");
+  HIP_TRY(zero_on(h->stream, h->status, 4 * sizeof(int)));
+  if (!h->gf) {
+    hipError_t e = capture(h, true, &h->gf);
+    if (e != hipSuccess) {
+      g_err = std::string("forward graph: ") + hipGetErrorString(e);
+      return HES_ERR_HIP;
+    }
+  }
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  HIP_TRY(hipGraphLaunch(h->gf, h->stream));
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (elapsed_ms) *elapsed_ms = ms;
+  int st[2];
+  HIP_TRY(copy_on(h->stream, st, h->status, sizeof(st), hipMemcpyDeviceToHost));
+  if (st[0]) return fail(HES_ERR_DOMAIN, "non-positive det(rho A) or non-finite state");
+  std::vector<double> lp(h->nbl);
+  HIP_TRY(copy_on(h->stream, lp.data(), h->lossp, sizeof(double) * h->nbl, hipMemcpyDeviceToHost));
+  double l = 0.0;
+  for (double v : lp) l += v;
+  if (loss) *loss = l;
+  h->fwd_done = true;
+  return HES_OK;
+}
+
+int hes_adjoint_backward(hes_adjoint* h, double* grad_states, double* grad_material,
+                         float* elapsed_ms) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  if (!h->fwd_done) return fail(HES_ERR_ARG, "run the forward pass first");
+  if (!h->gb) {
+    hipError_t e = capture(h, false, &h->gb);
+    if (e != hipSuccess) {
+      g_err = std::string("backward graph: ") + hipGetErrorString(e);
+      return HES_ERR_HIP;
+    }
+  }
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  HIP_TRY(hipGraphLaunch(h->gb, h->stream));
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (elapsed_ms) *elapsed_ms = ms;
+  std::vector<double> g((size_t)NP * h->nst + 2);
+  HIP_TRY(copy_on(h->stream, g.data(), h->gout, sizeof(double) * g.size(), hipMemcpyDeviceToHost));
+  if (grad_states) std::memcpy(grad_states, g.data(), sizeof(double) * NP * h->nst);
+  if (grad_material) {
+    grad_material[0] = g[NP * h->nst];
+    grad_material[1] = g[NP * h->nst + 1];
+  }
+  for (double v : g)
+    if (!std::isfinite(v)) return fail(HES_ERR_DOMAIN, "non-finite gradient");
+  return HES_OK;
+}
+
+int hes_adjoint_get_state(hes_adjoint* h, double* Q) {
+  if (!h || !Q) return fail(HES_ERR_ARG, "null argument");
+  if (!h->fwd_done) return fail(HES_ERR_ARG, "run the forward pass first");
+  double *qn, *cn;
+  fwd_slot(h, h->nsteps, &qn, &cn);
+  std::vector<double> buf((size_t)14 * h->pitch);
+  HIP_TRY(copy_on(h->stream, buf.data(), qn, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+  for (long long i = 0; i < h->n; ++i)
+    for (int r = 0; r < 14; ++r) Q[i * 14 + r] = buf[r * h->pitch + i];
+  return HES_OK;
+}
+
+int hes_adjoint_info(hes_adjoint* h, long long* device_bytes, long long* checkpoints) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  const long long P = h->pitch;
+  long long b = h->nck * 15 * P + (h->every - 1) * 14 * P + 2 * (14 + 1 + 14) * P;
+  b = b * 8 + (h->nsteps + 1) * 8 + h->nsteps * (long long)h->nb * 16 + h->nsteps * (8 + JL * 8);
+  if (device_bytes) *device_bytes = b;
+  if (checkpoints) *checkpoints = h->nck;
+  return HES_OK;
+}
+
+}  // extern "C"

@@ -55,7 +55,8 @@ __device__ __forceinline__ long long xcd_block() {
 // One 64-lane chunk: lane l holds cell i = first + l - 1 (lanes 0 and 63 are
 // halo cells); q and cc are its state and sound speeds.  Updates the owned
 // interior cells, writes their new state and sound speeds, and returns the
-// lane's CFL speed of the new state (0 for lanes that own nothing).
+// lane's CFL speed of the new state (0 for lanes that own nothing).  When
+// lam_cur is given it receives the CFL speed of the lane's current state.
 // Where a chunk reads and writes: the cells of one problem are [lo, hi) of the
 // structure-of-arrays state; its first and last cells are frozen.
 struct ChunkCtx {
@@ -71,7 +72,8 @@ struct ChunkCtx {
 
 template <int NPH, int FLUX, int FAST>
 __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, const double* q,
-                                             const double* cc, double lmax) {
+                                             const double* cc, double lmax,
+                                             double* lam_cur = nullptr) {
   constexpr int NC = NComp<NPH>::value;
   const int lane = threadIdx.x & 63;
   const long long pitch = a.pitch;
@@ -154,6 +156,7 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
 #pragma unroll
     for (int p = 0; p < NPH; ++p) st_stream(cc[p], a.Cout + p * pitch + i);
   }
+  if (lam_cur) *lam_cur = fmax(fabs(smin), fabs(smax));
   int anybad = (owned && bad) || bad2;
   if (__any(anybad) && lane == 0) atomicOr(a.status, 1);
   return lam_new;

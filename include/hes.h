@@ -125,6 +125,39 @@ int hes_batch_run(hes_batch* h, long long steps, int mode, float* elapsed_ms);
  * restored afterwards) and keep the fastest. */
 int hes_batch_autotune(hes_batch* h, int steps, int* block, int* xcd, float* ms_per_step);
 
+/* ---- Adjoint of a fixed-length single-phase Rusanov run (the reference has
+ *      no adjoint; its forward model is main.jl:199-225 with NumFluxes.jl:25-60
+ *      and HyperelasticityMPh.jl:63-175).  Loss of the final state:
+ *          L = sum_i (rho_i - rho*_i)^2 + (u1_i - u1*_i)^2
+ *      (rho, u1 as cons2prim gives them, HyperelasticityMPh.jl:106-121).
+ *      Gradients with respect to the primitive initial pieces (u(3), vec(A)(9),
+ *      S per piece; rho = rho0 / det A) and the material constants b0, c0.
+ *      Every `every`-th state is stored; the states in between are recomputed
+ *      by the backward pass.  Only alpha = 1, beta = 3, gamma = 2 materials. */
+typedef struct hes_adjoint hes_adjoint;
+
+/* material: the 8 Barton2009 constants; nsteps: steps of the run (no end time). */
+int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* material, double cfl,
+                       double dx, long long nsteps, long long every);
+int hes_adjoint_destroy(hes_adjoint* h);
+/* nstates pieces, prims[13 * k + (0:3 u, 3:12 vec(A), 12 S)], nstates-1 sorted
+ * interfaces (cell i takes piece #{j : (i+1/2)/ncells > s_j}, as
+ * hes_solver_set_piecewise). */
+int hes_adjoint_set_initial(hes_adjoint* h, int nstates, const double* prims,
+                            const double* interfaces);
+/* Target profiles rho*, u1* (ncells each). */
+int hes_adjoint_set_target(hes_adjoint* h, const double* rho, const double* u1);
+/* Forward run from the initial pieces: stores the checkpoints, returns L. */
+int hes_adjoint_forward(hes_adjoint* h, double* loss, float* elapsed_ms);
+/* Reverse pass after hes_adjoint_forward: grad_states[13 * nstates] (same
+ * layout as prims), grad_material[2] = (dL/db0, dL/dc0). */
+int hes_adjoint_backward(hes_adjoint* h, double* grad_states, double* grad_material,
+                         float* elapsed_ms);
+/* Final state of the forward run (14 x ncells, Julia layout). */
+int hes_adjoint_get_state(hes_adjoint* h, double* Q_host);
+/* Device memory held by the handle and the number of stored checkpoints. */
+int hes_adjoint_info(hes_adjoint* h, long long* device_bytes, long long* checkpoints);
+
 /* ---- Point-wise functions over n states (host arrays, synchronous). ---- */
 
 /* HyperelasticityMPh.cons2prim_mph (HyperelasticityMPh.jl:99-133):

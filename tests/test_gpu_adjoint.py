@@ -1,0 +1,190 @@
+"""Adjoint of the single-phase Rusanov run on the GPU.
+
+Checks: the adjoint's forward pass is the solver's run; gradients of the
+misfit with respect to the 13 primitives of both initial pieces and (b0, c0)
+agree with central differences (h = 1e-6 |theta_k|) of the GPU forward model
+and of the oracle's run (an independent CPU restatement of the reference
+loop); the checkpoint interval does not change a single bit of the result.
+"""
+import numpy as np
+import pytest
+
+from oracle_bind import rel_err_per_component
+
+pytestmark = pytest.mark.gpu
+
+SHEAR_UP, BULK_DOWN = 1.05, 0.97   # target run: b0 * 1.05, c0 * 0.97
+
+
+def _problem(n, steps, seed=7, every=None):
+    """Two seeded pieces split at x = 1/2 and the target profiles of a run
+    with perturbed shear and bulk speeds."""
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    from hyperelasticsolver_amd.adjoint import Adjoint, final_profiles, phase_prims_to_adjoint
+    from hyperelasticsolver_amd.initial import random_phase_prims
+    rng = np.random.default_rng(seed)
+    eos = Barton2009()
+    P14 = random_phase_prims(rng, eos, 2)
+    prims = phase_prims_to_adjoint(P14)
+    eos_t = Barton2009(_b0=eos.b0 * SHEAR_UP, _c0=eos.c0 * BULK_DOWN)
+    with Adjoint(eos_t, n, steps, every=every or steps) as t:
+        t.set_initial(prims, [0.5])
+        t.set_target(np.zeros(n), np.zeros(n))
+        t.forward()
+        rho_t, u_t = final_profiles(eos_t, t.final_state())
+    return eos, P14, prims, rho_t, u_t
+
+
+def _loss(prims, b0, c0, n, steps, rho_t, u_t):
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    from hyperelasticsolver_amd.adjoint import Adjoint
+    with Adjoint(Barton2009(_b0=b0, _c0=c0), n, steps, every=steps) as a:
+        a.set_initial(prims, [0.5])
+        a.set_target(rho_t, u_t)
+        return a.forward()
+
+
+def _gradient(eos, prims, rho_t, u_t, n, steps, every):
+    from hyperelasticsolver_amd.adjoint import Adjoint
+    with Adjoint(eos, n, steps, every=every) as a:
+        a.set_initial(prims, [0.5])
+        a.set_target(rho_t, u_t)
+        L = a.forward()
+        g, gm = a.backward()
+    return L, np.concatenate([g[:, 0], g[:, 1], gm])
+
+
+def _params(prims, eos):
+    return np.concatenate([prims[:, 0], prims[:, 1], [eos.b0, eos.c0]])
+
+
+def _unpack(theta):
+    return np.asfortranarray(np.stack([theta[:13], theta[13:26]], axis=1)), theta[26], theta[27]
+
+
+def test_adjoint_forward_is_the_solver_run(ora):
+    from hyperelasticsolver_amd.adjoint import Adjoint
+    from hyperelasticsolver_amd.initial import prim2cons
+    from hyperelasticsolver_amd.solver import Solver
+    print("[adjoint] forward pass vs solver run", flush=True)
+    n, steps = 700, 150
+    eos, P14, prims, rho_t, u_t = _problem(n, steps, seed=3)
+    with Adjoint(eos, n, steps, every=13) as a:
+        a.set_initial(prims, [0.5])
+        a.set_target(rho_t, u_t)
+        a.forward()
+        Qa = a.final_state()
+    with Solver(eos, n, flux="rusanov", cfl=0.6) as s:
+        s.set_piecewise(prim2cons(eos, P14), [0.5])
+        s.run(steps=steps, use_graph=False)
+        Qs = s.get_state()
+    assert np.all(rel_err_per_component(Qa, Qs) <= 1e-10)
+
+
+def test_checkpoint_interval_changes_nothing():
+    print("[adjoint] checkpoint interval invariance", flush=True)
+    n, steps = 400, 90
+    eos, _, prims, rho_t, u_t = _problem(n, steps, seed=5)
+    ref = None
+    for every in (1, 7, 45, 90):
+        L, g = _gradient(eos, prims, rho_t, u_t, n, steps, every)
+        assert np.all(np.isfinite(g))
+        if ref is None:
+            ref = (L, g)
+        else:
+            assert L == ref[0]
+            assert np.array_equal(g, ref[1]), every
+
+
+def _check(g, fd, rel, what):
+    scale = np.abs(fd).max()
+    for k in range(len(g)):
+        tol = rel * max(abs(fd[k]), abs(g[k])) + 1e-9 * scale
+        assert abs(g[k] - fd[k]) <= tol, (what, k, g[k], fd[k])
+
+
+def _step(theta_k, rel):
+    """Difference step relative to the parameter: the entropy S (1e-3) enters
+    as exp(S / cv) with cv = 3.9e-4, so an absolute 1e-6 would be 0.26% of its
+    scale and the central difference would carry a 5e-4 truncation error."""
+    return rel * abs(theta_k) if theta_k != 0 else rel
+
+
+def test_gradient_matches_central_differences():
+    """28 gradients (u, A, S of both pieces; b0, c0) within 1e-6 of central
+    differences of the GPU forward model, h = 1e-6 |theta_k|."""
+    print("[adjoint] gradient vs central differences (h = 1e-6 |theta|)", flush=True)
+    n, steps = 320, 160
+    eos, _, prims, rho_t, u_t = _problem(n, steps, seed=11)
+    L, g = _gradient(eos, prims, rho_t, u_t, n, steps, every=9)
+    assert L > 0
+    theta = _params(prims, eos)
+    fd = np.empty(28)
+    for k in range(28):
+        h = _step(theta[k], 1e-6)
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd[k] = (_loss(*_unpack(tp), n, steps, rho_t, u_t) -
+                 _loss(*_unpack(tm), n, steps, rho_t, u_t)) / (2 * h)
+    _check(g, fd, 1e-6, "gpu-fd")
+
+
+def test_gradient_matches_oracle_differences(ora):
+    """The same 28 gradients against central differences (h = 1e-6 |theta_k|)
+    of the oracle's run at 512 cells.  The misfit is strongly curved here (the
+    one-sided quotients of A_22 differ by 0.3% at h = 1e-7), so larger steps or
+    Richardson extrapolation from h = 1e-4 are outside the asymptotic range."""
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    print("[adjoint] gradient vs oracle differences at 512 cells", flush=True)
+    n, steps = 512, 120
+    eos, _, prims, rho_t, u_t = _problem(n, steps, seed=13)
+    L, g = _gradient(eos, prims, rho_t, u_t, n, steps, every=25)
+    theta = _params(prims, eos)
+
+    def oracle_loss(th):
+        pr, b0, c0 = _unpack(th)
+        e = Barton2009(_b0=b0, _c0=c0)
+        mats = e.params()
+        P = np.zeros((14, 2), order="F")
+        for j in range(2):
+            A = pr[3:12, j].reshape(3, 3, order="F")
+            P[0, j] = e.rho0 / np.linalg.det(A)
+            P[1:4, j] = pr[0:3, j]
+            P[4, j] = pr[12, j]
+            P[5:14, j] = pr[3:12, j]
+        Qs = ora.prim2cons(1, mats, P)
+        x = (np.arange(n) + 0.5) / n
+        Q0 = np.asfortranarray(Qs[:, (x > 0.5).astype(int)])
+        Q, _, s, _ = ora.run(1, mats, 0, Q0, 0.6, 1.0 / n, steps=steps)
+        assert s == steps
+        Pn = ora.cons2prim(1, mats, Q)
+        return float(np.sum((Pn[0] - rho_t) ** 2) + np.sum((Pn[1] - u_t) ** 2))
+
+    assert oracle_loss(theta) == pytest.approx(L, rel=1e-9)
+    fd = np.empty(28)
+    for k in range(28):
+        h = _step(theta[k], 1e-6)
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd[k] = (oracle_loss(tp) - oracle_loss(tm)) / (2 * h)
+    _check(g, fd, 1e-6, "oracle-fd")
+
+
+def test_full_size_gradient_spot_check():
+    """2^18 cells, 2000 steps, checkpoints every 50: b0, c0, u1 and S of the
+    left piece and A11 of the right one against central differences."""
+    print("[adjoint] 2^18 cells x 2000 steps spot check", flush=True)
+    n, steps = 1 << 18, 2000
+    eos, _, prims, rho_t, u_t = _problem(n, steps, seed=2, every=50)
+    L, g = _gradient(eos, prims, rho_t, u_t, n, steps, every=50)
+    theta = _params(prims, eos)
+    for k in (26, 27, 0, 13 + 3, 12):
+        h = _step(theta[k], 1e-6)
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (_loss(*_unpack(tp), n, steps, rho_t, u_t) -
+              _loss(*_unpack(tm), n, steps, rho_t, u_t)) / (2 * h)
+        assert abs(g[k] - fd) <= 1e-6 * max(abs(fd), abs(g[k])) + 1e-9 * abs(g).max(), (k, g[k], fd)
