@@ -111,6 +111,71 @@ def time_batched(steps, loop_steps=5):
             "autotune": tuned}
 
 
+def adjoint_problem(n, steps, every, seed=2):
+    """Two seeded pieces split at x = 1/2 (A = I + 0.05 N(0,1), u ~ U(-1/2,1/2)^3,
+    S = 1e-3) and the target profiles of a run with b0 * 1.05 and c0 * 0.97."""
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    from hyperelasticsolver_amd.adjoint import Adjoint, phase_prims_to_adjoint
+    from hyperelasticsolver_amd.initial import random_phase_prims
+    rng = np.random.default_rng(seed)
+    eos = Barton2009()
+    P14 = random_phase_prims(rng, eos, 2)
+    prims = phase_prims_to_adjoint(P14)
+    eos_t = Barton2009(_b0=eos.b0 * 1.05, _c0=eos.c0 * 0.97)
+    with Adjoint(eos_t, n, steps, every=every) as t:
+        t.set_initial(prims, [0.5])
+        t.set_target(np.zeros(n), np.zeros(n))
+        t.forward()
+        rho_t, u_t = t.profiles()
+    return eos, P14, prims, rho_t, u_t
+
+
+def time_adjoint(n=1 << 18, steps=2000, every=50):
+    """(forward + adjoint) / forward: the plain forward is the solver's step
+    loop on the same problem (one launch per step); the adjoint run is its
+    checkpointed forward pass plus the reverse pass (recompute + reverse
+    steps).  Reported for each launch mode, with zero-tile skipping (the
+    default) and without it (every tile of every reverse step evaluated)."""
+    from hyperelasticsolver_amd.adjoint import Adjoint
+    from hyperelasticsolver_amd.initial import prim2cons
+    from hyperelasticsolver_amd.solver import Solver
+    eos, P14, prims, rho_t, u_t = adjoint_problem(n, steps, every)
+    with Solver(eos, n, flux="rusanov", cfl=0.6) as s:
+        s.set_piecewise(prim2cons(eos, P14), [0.5])
+        s.time_steps(50)
+        s.set_piecewise(prim2cons(eos, P14), [0.5])
+        fwd_ms = s.time_steps(steps)
+    runs = {}
+    grads = []
+    with Adjoint(eos, n, steps, every=every) as a:
+        a.set_initial(prims, [0.5])
+        a.set_target(rho_t, u_t)
+        info = a.info()
+        for graph in (True, False):
+            for sparse in (True, False):
+                a.set_launch(graph)
+                a.set_sparse(sparse)
+                a.forward()
+                a.backward()          # the first calls capture the graphs
+                loss = a.forward()
+                g, gm = a.backward()
+                grads.append(np.concatenate([g.ravel(), gm]))
+                runs[f"{'graph' if graph else 'eager'}_{'sparse' if sparse else 'dense'}"] = {
+                    "forward_ms": a.last_forward_ms, "backward_ms": a.last_backward_ms,
+                    "ratio": (a.last_forward_ms + a.last_backward_ms) / fwd_ms}
+    if not all(np.array_equal(grads[0], x) for x in grads[1:]):
+        raise RuntimeError("launch modes disagree")
+    if not np.all(np.isfinite(grads[0])):
+        raise RuntimeError("non-finite gradient")
+    best = min((v["ratio"], k) for k, v in runs.items() if k.endswith("sparse"))[1]
+    dense = min((v["ratio"], k) for k, v in runs.items() if k.endswith("dense"))[1]
+    return {"cells": n, "steps": steps, "checkpoint_every": every, "loss": loss,
+            "forward_ms": fwd_ms, "fwd_plus_adj_over_fwd": runs[best]["ratio"], "mode": best,
+            "fwd_plus_adj_over_fwd_dense": runs[dense]["ratio"], "mode_dense": dense,
+            "runs": runs, "device_bytes": info["device_bytes"], "checkpoints": info["checkpoints"],
+            "grad_b0_c0": [float(gm[0]), float(gm[1])]}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +229,7 @@ def main():
         out[other] = {"value": v2, "ms_per_step": ms2 / args.steps,
                       "hbm_frac": BYTES_PER_CELL_1PH * v2 / 1e9 / HBM_PEAK_GBS}
         out["batched"] = time_batched(args.batched_steps)
+        out["adjoint"] = time_adjoint()
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
         out["cpu_baseline"] = cpu_baseline(eos, Qs, ifc, 0 if args.flux == "rusanov" else 1, threads)

@@ -64,7 +64,10 @@ _SIGS = {
     "hes_adjoint_set_target": [_vp, _dp, _dp],
     "hes_adjoint_forward": [_vp, _dp, ctypes.POINTER(ctypes.c_float)],
     "hes_adjoint_backward": [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_float)],
+    "hes_adjoint_set_sparse": [_vp, _i],
+    "hes_adjoint_set_launch": [_vp, _i],
     "hes_adjoint_get_state": [_vp, _dp],
+    "hes_adjoint_get_profiles": [_vp, _dp, _dp],
     "hes_adjoint_info": [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_ll)],
     "hes_batch_create": [ctypes.POINTER(_vp), _i, _i, _i, ctypes.POINTER(_ll), _dp, _dp, _dp],
     "hes_batch_destroy": [_vp],

@@ -111,24 +111,30 @@ class Adjoint:
         self.last_backward_ms = ms.value
         return g, gm
 
+    def set_sparse(self, on=True):
+        """Skip reverse-step tiles whose cotangent is exactly zero (default on;
+        the gradients are the same either way)."""
+        check(lib().hes_adjoint_set_sparse(self._h, 1 if on else 0))
+
+    def set_launch(self, use_graph=True):
+        """Replay each pass as one hipGraph (default) or launch its kernels one by one."""
+        check(lib().hes_adjoint_set_launch(self._h, 1 if use_graph else 0))
+
     def final_state(self):
         Q = np.empty((14, self.n), order="F")
         check(lib().hes_adjoint_get_state(self._h, ptr(Q)))
         return Q
+
+    def profiles(self):
+        """(rho, u1) of the final state as the misfit computes them: the
+        target profiles of a run made to be matched by another."""
+        r = np.empty(self.n)
+        u = np.empty(self.n)
+        check(lib().hes_adjoint_get_profiles(self._h, ptr(r), ptr(u)))
+        return r, u
 
     def info(self):
         b = ctypes.c_longlong()
         c = ctypes.c_longlong()
         check(lib().hes_adjoint_info(self._h, ctypes.byref(b), ctypes.byref(c)))
         return {"device_bytes": b.value, "checkpoints": c.value}
-
-
-def final_profiles(eos, Q):
-    """rho and u1 of every cell of a single-phase state (cons2prim,
-    HyperelasticityMPh.jl:106-121)."""
-    from ._lib import columns
-    q, n, _ = columns(Q, 14, "Q")
-    P = np.empty((14, n), order="F")
-    mats = np.ascontiguousarray(materials_array(eos), dtype=np.float64)
-    check(lib().hes_cons2prim(1, ptr(mats), ptr(q), ptr(P), n))
-    return P[0].copy(), P[1].copy()

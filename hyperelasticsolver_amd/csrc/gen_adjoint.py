@@ -277,16 +277,30 @@ def trace_prim2cons():
     return g, u + A + [S, b0, c0, rho0, cv, t0], Q
 
 
+def observed(Q, rho0):
+    """rho and u1 of a cell as cons2prim gives them (HyperelasticityMPh.jl:113-116)."""
+    rho = sqrt(det3(Q[5:14]) * (1.0 / rho0))
+    return rho, Q[1] * (1.0 / rho)
+
+
 def trace_loss():
     """in: Q(14), rho*, u1*, rho0 -> out: (rho - rho*)^2 + (u1 - u1*)^2 with rho, u1 from cons2prim."""
     g = Graph()
     Q = [g.inp(f"q{k}") for k in range(14)]
     rt, ut, rho0 = g.inp("rho_t"), g.inp("u_t"), g.inp("rho0")
-    rho = sqrt(det3(Q[5:14]) * (1.0 / rho0))
-    u1 = Q[1] * (1.0 / rho)
+    rho, u1 = observed(Q, rho0)
     d1 = rho - rt
     d2 = u1 - ut
     return g, Q + [rt, ut, rho0], [d1 * d1 + d2 * d2]
+
+
+def trace_profiles():
+    """in: Q(14), rho0 -> out: rho, u1 (the observed quantities of the misfit)."""
+    g = Graph()
+    Q = [g.inp(f"q{k}") for k in range(14)]
+    rho0 = g.inp("rho0")
+    rho, u1 = observed(Q, rho0)
+    return g, Q + [rho0], [rho, u1]
 
 
 # ----------------------------------------------------------------------------- emission
@@ -295,17 +309,29 @@ def _lit(x):
     return r if ("." in r or "e" in r or "inf" in r or "nan" in r) else r + ".0"
 
 
-def emit(fname, g, inputs, outputs):
-    """C++ for `fname_fwd(in, out)` and `fname_vjp(in, obar, ibar)` (ibar accumulates)."""
-    idx_in = {v.i: k for k, v in enumerate(inputs)}
-    live = set()  # nodes reachable from outputs
-    stack = [o.i for o in outputs if o is not None]
+def _ancestors(g, roots):
+    seen = set()
+    stack = [o.i for o in roots if o is not None]
     while stack:
         n = stack.pop()
-        if n in live:
+        if n in seen:
             continue
-        live.add(n)
+        seen.add(n)
         stack.extend(a.i for a in g.nodes[n][1] if isinstance(a, Var))
+    return seen
+
+
+def emit(fname, g, inputs, outputs, fused_split=None, exact=False):
+    """C++ for `fname_fwd(in, out)` and `fname_vjp(in, obar, ibar)` (ibar accumulates).
+
+    With fused_split = k, also `fname_fwd_vjp(in, out, seed, ibar)`: the primal,
+    then obar from `seed(out, obar)` (which returns whether obar[k:] may be
+    non-zero), then the reverse sweep.  The nodes that only feed outputs k..
+    are swept first, inside `if`, so a zero tail of obar skips them: none of
+    them is an ancestor of a node that feeds outputs 0..k-1, so their
+    adjoints are complete before the shared nodes are reached."""
+    idx_in = {v.i: k for k, v in enumerate(inputs)}
+    live = _ancestors(g, outputs)  # nodes reachable from outputs
     order = sorted(live)
 
     def val(a):
@@ -335,48 +361,71 @@ def emit(fname, g, inputs, outputs):
     nout = len(outputs)
     outs_fwd = [f"  out[{k}] = {val(o) if o is not None else '0.0'};" for k, o in enumerate(outputs)]
 
-    rev = [f"  double g{n} = 0.0;" for n in order if g.nodes[n][0] != "const"]
-    for k, o in enumerate(outputs):
-        if o is not None and g.nodes[o.i][0] != "const":
-            rev.append(f"  g{o.i} += obar[{k}];")
-    for n in reversed(order):
+    def is_var(a):
+        return g.nodes[a.i][0] != "const"
+
+    def rev_node(n):
         op, args, _ = g.nodes[n]
         if op in ("in", "const"):
-            continue
+            return []
         gn = f"g{n}"
-        tgt = [a for a in args if isinstance(a, Var) and g.nodes[a.i][0] != "const"]
-        is_var = lambda a: g.nodes[a.i][0] != "const"
         a0 = args[0]
         a1 = args[1] if len(args) > 1 else None
+        r = []
         if op == "add":
-            if is_var(a0): rev.append(f"  g{a0.i} += {gn};")
-            if is_var(a1): rev.append(f"  g{a1.i} += {gn};")
+            if is_var(a0): r.append(f"  g{a0.i} += {gn};")
+            if is_var(a1): r.append(f"  g{a1.i} += {gn};")
         elif op == "sub":
-            if is_var(a0): rev.append(f"  g{a0.i} += {gn};")
-            if is_var(a1): rev.append(f"  g{a1.i} -= {gn};")
+            if is_var(a0): r.append(f"  g{a0.i} += {gn};")
+            if is_var(a1): r.append(f"  g{a1.i} -= {gn};")
         elif op == "mul":
-            if is_var(a0): rev.append(f"  g{a0.i} += {gn} * {val(a1)};")
-            if is_var(a1): rev.append(f"  g{a1.i} += {gn} * {val(a0)};")
+            if is_var(a0): r.append(f"  g{a0.i} += {gn} * {val(a1)};")
+            if is_var(a1): r.append(f"  g{a1.i} += {gn} * {val(a0)};")
         elif op == "div":
-            if is_var(a0): rev.append(f"  g{a0.i} += {gn} / {val(a1)};")
-            if is_var(a1): rev.append(f"  g{a1.i} -= {gn} * v{n} / {val(a1)};")
+            if is_var(a0): r.append(f"  g{a0.i} += {gn} / {val(a1)};")
+            if is_var(a1): r.append(f"  g{a1.i} -= {gn} * v{n} / {val(a1)};")
         elif op == "neg":
-            rev.append(f"  g{a0.i} -= {gn};")
+            r.append(f"  g{a0.i} -= {gn};")
         elif op == "sqrt":
-            rev.append(f"  g{a0.i} += {gn} * (0.5 / v{n});")
+            r.append(f"  g{a0.i} += {gn} * (0.5 / v{n});")
         elif op == "exp":
-            rev.append(f"  g{a0.i} += {gn} * v{n};")
+            r.append(f"  g{a0.i} += {gn} * v{n};")
         elif op == "clampmin":
-            rev.append(f"  if (!({val(a0)} < {val(a1)})) g{a0.i} += {gn};")
-        del tgt
+            r.append(f"  if (!({val(a0)} < {val(a1)})) g{a0.i} += {gn};")
+        return r
+
+    init = [f"  double g{n} = 0.0;" for n in order if g.nodes[n][0] != "const"]
+    seeds = [f"  g{o.i} += obar[{k}];" for k, o in enumerate(outputs)
+             if o is not None and g.nodes[o.i][0] != "const"]
+    rev = []
+    for n in reversed(order):
+        rev += rev_node(n)
     rev_out = [f"  ibar[{k}] += g{v.i};" for k, v in enumerate(inputs) if v.i in live]
     nin = len(inputs)
     head = f"// {fname}: {nin} inputs, {nout} outputs, {len(order)} SSA values\n"
-    f1 = (f"HES_HD inline void {fname}_fwd(const double* in, double* out) {{\n" + "\n".join(fwd) + "\n" +
+    # exact: no contraction into fused multiply-adds, so every kernel that
+    # inlines the routine rounds it the same way
+    pre = "#pragma clang fp contract(off)\n" if exact else ""
+    f1 = (f"HES_HD inline void {fname}_fwd(const double* in, double* out) {{\n" + pre + "\n".join(fwd) + "\n" +
           "\n".join(outs_fwd) + "\n}\n")
-    f2 = (f"HES_HD inline void {fname}_vjp(const double* in, const double* obar, double* ibar) {{\n" +
-          "\n".join(fwd) + "\n" + "\n".join(rev) + "\n" + "\n".join(rev_out) + "\n}\n")
-    return head + f1 + "\n" + f2
+    f2 = (f"HES_HD inline void {fname}_vjp(const double* in, const double* obar, double* ibar) {{\n" + pre +
+          "\n".join(fwd) + "\n" + "\n".join(init + seeds + rev) + "\n" + "\n".join(rev_out) + "\n}\n")
+    text = head + f1 + "\n" + f2
+    if fused_split is not None:
+        head_set = _ancestors(g, outputs[:fused_split])
+        tail_only = [n for n in reversed(order) if n not in head_set]
+        rest = [n for n in reversed(order) if n in head_set]
+        rev_tail = [r for n in tail_only for r in rev_node(n)]
+        rev_rest = [r for n in rest for r in rev_node(n)]
+        f3 = (f"template <class Seed>\n"
+              f"HES_HD inline void {fname}_fwd_vjp(const double* in, double* out, Seed&& seed, "
+              f"double* ibar) {{\n" + "\n".join(fwd) + "\n" + "\n".join(outs_fwd) + "\n"
+              f"  double obar[{nout}];\n"
+              f"  const bool tail = seed(static_cast<const double*>(out), obar);\n" +
+              "\n".join(init + seeds) + "\n  if (tail) {\n" + "\n".join("  " + r for r in rev_tail) +
+              "\n  }\n" + "\n".join(rev_rest) + "\n" + "\n".join(rev_out) + "\n}\n")
+        text += "\n" + f3
+    return text
 
 
 def main(path=os.path.join(HERE, "hes_adjoint_gen.hpp")):
@@ -384,15 +433,18 @@ def main(path=os.path.join(HERE, "hes_adjoint_gen.hpp")):
         "// hes_adjoint_gen.hpp -- GENERATED by gen_adjoint.py; do not edit.\n"
         "// Primal and reverse-mode (VJP) routines of the single-phase per-cell physics.\n"
         "#pragma once\n#include <cmath>\n"
+        "#if !defined(__clang__)\n#pragma GCC diagnostic ignored \"-Wunknown-pragmas\"\n#endif\n"
         "#if defined(__HIPCC__)\n#define HES_HD __host__ __device__\n#else\n#define HES_HD\n#endif\n"
         "namespace hes_gen {\nusing std::sqrt;\nusing std::exp;\n"
-        "constexpr int CELL_IN = 19, CELL_OUT = 21, P2C_IN = 18, P2C_OUT = 14, LOSS_IN = 17;\n"]
+        "constexpr int CELL_IN = 19, CELL_OUT = 21, P2C_IN = 18, P2C_OUT = 14, LOSS_IN = 17, PROF_IN = 15;\n"]
     g, i, o = trace_cell()
-    parts.append(emit("cell", g, i, o))
+    parts.append(emit("cell", g, i, o, fused_split=14))
     g, i, o = trace_prim2cons()
     parts.append(emit("prim2cons", g, i, o))
     g, i, o = trace_loss()
     parts.append(emit("loss", g, i, o))
+    g, i, o = trace_profiles()
+    parts.append(emit("profiles", g, i, o, exact=True))
     parts.append("}  // namespace hes_gen\n")
     text = "\n".join(parts)
     with open(path, "w") as f:

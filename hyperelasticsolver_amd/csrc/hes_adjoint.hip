@@ -71,21 +71,6 @@ __device__ __forceinline__ double block_sum(double v) {
   return s;
 }
 
-__device__ __forceinline__ void block_max_atomic(double v, unsigned long long* slot) {
-  __shared__ double red[ABLK / 64];
-  v = wave_max(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = red[0];
-#pragma unroll
-    for (int k = 1; k < ABLK / 64; ++k) m = fmax(m, red[k]);
-    if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
-  }
-}
-
 // (value, index) with the larger value, ties to the smaller index.
 __device__ __forceinline__ void arg_better(double& v, long long& i, double ov, long long oi) {
   if (ov > v || (ov == v && oi < i)) {
@@ -94,10 +79,11 @@ __device__ __forceinline__ void arg_better(double& v, long long& i, double ov, l
   }
 }
 
-__device__ __forceinline__ void block_argmax(double v, long long idx, double* out_v,
-                                             long long* out_i) {
-  __shared__ double rv[ABLK / 64];
-  __shared__ long long ri[ABLK / 64];
+// Block argmax of (v, idx) over NT threads; the result is valid in thread 0.
+template <int NT>
+__device__ __forceinline__ void block_argmax(double& v, long long& idx) {
+  __shared__ double rv[NT / 64];
+  __shared__ long long ri[NT / 64];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     double ov = __shfl_xor(v, o);
@@ -111,12 +97,33 @@ __device__ __forceinline__ void block_argmax(double v, long long idx, double* ou
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double bv = rv[0];
-    long long bi = ri[0];
 #pragma unroll
-    for (int k = 1; k < ABLK / 64; ++k) arg_better(bv, bi, rv[k], ri[k]);
-    out_v[blockIdx.x] = bv;
-    out_i[blockIdx.x] = bi;
+    for (int k = 1; k < NT / 64; ++k) arg_better(v, idx, rv[k], ri[k]);
+  }
+}
+
+constexpr long long NO_CELL = 0x7fffffffffffffffLL;
+
+// Max CFL speed of a state (into lmax, as double bits) and, optionally, the
+// per-block argmax (value, smallest index) of the owned cells' speeds.
+template <int NT>
+__device__ __forceinline__ void record_speeds(double lam, bool owned, long long i,
+                                              unsigned long long* lmax, double* bv,
+                                              long long* bi) {
+  double v = owned ? lam : -1.0;
+  long long idx = owned ? i : NO_CELL;
+  block_argmax<NT>(v, idx);
+  if (threadIdx.x == 0) {
+    // read first: most blocks find a larger maximum already there
+    if (lmax && v > 0.0) {
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+      if (bits > __hip_atomic_load(lmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(lmax, bits);
+    }
+    if (bv) {
+      bv[blockIdx.x] = v;
+      bi[blockIdx.x] = idx;
+    }
   }
 }
 
@@ -226,7 +233,8 @@ __global__ void k_adj_zero(double* __restrict__ p, long long m) {
 __global__ __launch_bounds__(LBLK) void k_adj_speeds(const double* __restrict__ Q,
                                                      double* __restrict__ C, long long n,
                                                      long long pitch, MatDev m,
-                                                     unsigned long long* lmax0, int* status) {
+                                                     unsigned long long* lmax0, double* bv,
+                                                     long long* bi, int* status) {
   long long i = (long long)blockIdx.x * LBLK + threadIdx.x;
   double lam = 0.0;
   int bad = 0;
@@ -238,7 +246,7 @@ __global__ __launch_bounds__(LBLK) void k_adj_speeds(const double* __restrict__ 
     C[i] = cc[0];
   }
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1);
-  block_max_atomic(lam, lmax0);
+  record_speeds<LBLK>(lam, i < n, i, lmax0, bv, bi);
 }
 
 struct AdjFwdArgs {
@@ -247,7 +255,7 @@ struct AdjFwdArgs {
   const double* __restrict__ Cin;
   double* __restrict__ Cout;
   unsigned long long* lmax;  // lmax[s] = max CFL speed of state s (double bits)
-  double* blk_val;           // optional: per-block argmax of the CFL speed of state s
+  double* blk_val;           // optional: per-block argmax of the CFL speed of state s + 1
   long long* blk_idx;
   int* status;
   long long n, pitch, s;
@@ -267,25 +275,25 @@ __global__ __launch_bounds__(ABLK) __attribute__((amdgpu_waves_per_eu(3))) void 
   double q[NQ], cc[1];
   load_chunk<1>(ctx, i, q, cc);
   const double lmax = lmax_of(a.lmax, a.s);
-  double lam_cur;
-  const double lam = chunk_step<1, FLUX_RUSANOV, 1>(ctx, i, q, cc, lmax, &lam_cur);
-  if (a.write_lmax) block_max_atomic(lam, a.lmax + a.s + 1);
-  if (a.blk_val) {
+  const double lam = chunk_step<1, FLUX_RUSANOV, 1>(ctx, i, q, cc, lmax);
+  if (a.write_lmax) {
     const bool owned = i >= 0 && i < a.n && lane >= 1 && lane <= TILE;
-    block_argmax(owned ? lam_cur : -1.0, owned ? i : (long long)0x7fffffffffffffffLL,
-                 a.blk_val + a.s * (long long)gridDim.x, a.blk_idx + a.s * (long long)gridDim.x);
+    const long long row = (a.s + 1) * (long long)gridDim.x;
+    record_speeds<ABLK>(lam, owned, i, a.lmax + a.s + 1, a.blk_val ? a.blk_val + row : nullptr,
+                        a.blk_idx ? a.blk_idx + row : nullptr);
   }
 }
 
 // Argmax cell of the CFL speed of every state from the per-block records.
 __global__ __launch_bounds__(64) void k_adj_argmax(const double* __restrict__ blk_val,
                                                    const long long* __restrict__ blk_idx, int nb,
-                                                   long long* __restrict__ istar) {
+                                                   int nb0, long long* __restrict__ istar) {
   const long long s = blockIdx.x;
   const int lane = threadIdx.x;
+  const int cnt = s == 0 ? nb0 : nb;  // state 0 was recorded by the per-cell speeds kernel
   double v = -2.0;
-  long long i = 0x7fffffffffffffffLL;
-  for (int b = lane; b < nb; b += 64) arg_better(v, i, blk_val[s * nb + b], blk_idx[s * nb + b]);
+  long long i = NO_CELL;
+  for (int b = lane; b < cnt; b += 64) arg_better(v, i, blk_val[s * nb + b], blk_idx[s * nb + b]);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     double ov = __shfl_xor(v, o);
@@ -305,24 +313,41 @@ __global__ __launch_bounds__(LBLK) void k_adj_loss(const double* __restrict__ Q,
   long long i = (long long)blockIdx.x * LBLK + threadIdx.x;
   double l = 0.0;
   if (i < n) {
-    double in[hes_gen::LOSS_IN];
+    // rho and u1 by the routine that also makes target profiles, so a cell
+    // where the runs agree has a misfit and a gradient of exactly zero
+    double in[hes_gen::PROF_IN], o[2];
 #pragma unroll
     for (int r = 0; r < NQ; ++r) in[r] = Q[r * pitch + i];
-    in[NQ] = rho_t[i];
-    in[NQ + 1] = u_t[i];
-    in[NQ + 2] = rho0;
-    double out[1];
-    hes_gen::loss_fwd(in, out);
-    l = out[0];
-    double ob[1] = {1.0}, ib[hes_gen::LOSS_IN];
+    in[NQ] = rho0;
+    hes_gen::profiles_fwd(in, o);
+    const double d1 = o[0] - rho_t[i], d2 = o[1] - u_t[i];
+    l = d1 * d1 + d2 * d2;
+    double ob[2] = {2.0 * d1, 2.0 * d2}, ib[hes_gen::PROF_IN];
 #pragma unroll
-    for (int r = 0; r < hes_gen::LOSS_IN; ++r) ib[r] = 0.0;
-    hes_gen::loss_vjp(in, ob, ib);
+    for (int r = 0; r < hes_gen::PROF_IN; ++r) ib[r] = 0.0;
+    hes_gen::profiles_vjp(in, ob, ib);
 #pragma unroll
     for (int r = 0; r < NQ; ++r) Qb[r * pitch + i] = ib[r];
   }
   double s = block_sum(l);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// rho and u1 of every cell with the misfit's own expressions (a target taken
+// from a run's profiles then matches it exactly wherever the runs agree).
+__global__ __launch_bounds__(LBLK) void k_adj_profiles(const double* __restrict__ Q, double rho0,
+                                                       double* __restrict__ rho,
+                                                       double* __restrict__ u1, long long n,
+                                                       long long pitch) {
+  long long i = (long long)blockIdx.x * LBLK + threadIdx.x;
+  if (i >= n) return;
+  double in[hes_gen::PROF_IN], o[2];
+#pragma unroll
+  for (int r = 0; r < NQ; ++r) in[r] = Q[r * pitch + i];
+  in[NQ] = rho0;
+  hes_gen::profiles_fwd(in, o);
+  rho[i] = o[0];
+  u1[i] = o[1];
 }
 
 // ---------------------------------------------------------------------------
@@ -370,6 +395,8 @@ struct AdjStepArgs {
   const double* __restrict__ Q;      // state s
   const double* __restrict__ Qb_in;  // Qbar of state s+1 (without its lmax term)
   double* __restrict__ Qb_out;       // Qbar of state s (without its lmax term)
+  const int* zin;                    // per tile: 1 if Qb_in is zero on all its owned cells
+  int* zout;                         // the same for Qb_out
   const unsigned long long* lmax;
   const long long* istar;
   const double* Jlam;
@@ -378,29 +405,46 @@ struct AdjStepArgs {
   double* thb;             // running (b0bar, c0bar) per block
   long long n, pitch, s;
   double cfl, dx;
-  int nb, has_next;
+  int nb, nw, has_next;
+  int sparse;              // skip tiles whose cotangent is zero on them and their halo
   double gin[5];           // b0, c0, rho0, cv, t0
 };
 
+// One reverse step.  A tile whose Qbar of state s+1 is zero on its owned
+// cells and both halo cells (and which holds no argmax cell of step s+1) has
+// Qbar of state s = 0 and contributes nothing to dtdxbar or (b0bar, c0bar):
+// every seed of its cell VJPs is zero, and the VJP is linear in its seeds.
 __global__ __launch_bounds__(ABLK) void k_adj_step(AdjStepArgs a) {
   const int lane = threadIdx.x & 63;
   const long long wave = (xcd_block() * ABLK + threadIdx.x) >> 6;
   const long long i = wave * TILE + lane - 1;
   const bool inr = i >= 0 && i < a.n;
   const bool owned = inr && lane >= 1 && lane <= TILE;
-  const long long ic = i < 0 ? 0 : (i >= a.n ? a.n - 1 : i);
   const long long pitch = a.pitch;
-  double q[NQ], qb[NQ];
+  const long long is1 = a.has_next ? a.istar[a.s + 1] : -2;
+  const bool hit = is1 >= wave * TILE - 1 && is1 <= wave * TILE + TILE;  // wave-uniform
+  bool skip = wave >= a.nw;  // tiles past the last cell own nothing
+  if (!skip && a.sparse && !hit) {
+    skip = a.zin[wave] && (wave == 0 || a.zin[wave - 1]) && (wave + 1 >= a.nw || a.zin[wave + 1]);
+    if (skip) {
+      if (!a.zout[wave] && owned) {
 #pragma unroll
-  for (int r = 0; r < NQ; ++r) {
-    q[r] = ld_stream(a.Q + r * pitch + ic);
-    qb[r] = inr ? ld_stream(a.Qb_in + r * pitch + ic) : 0.0;
+        for (int r = 0; r < NQ; ++r) a.Qb_out[r * pitch + i] = 0.0;
+      }
+      if (lane == 0) a.zout[wave] = 1;
+    }
   }
-  double th0 = 0.0, th1 = 0.0;
-  // lmax term of step s+1 at its argmax cell
-  if (a.has_next) {
-    const long long is1 = a.istar[a.s + 1];
-    if (__any(inr && i == is1)) {
+  double dts = 0.0, th0 = 0.0, th1 = 0.0;
+  if (!skip) {
+    const long long ic = i < 0 ? 0 : (i >= a.n ? a.n - 1 : i);
+    double q[NQ], qb[NQ];
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) {
+      q[r] = ld_stream(a.Q + r * pitch + ic);
+      qb[r] = inr ? ld_stream(a.Qb_in + r * pitch + ic) : 0.0;
+    }
+    // lmax term of step s+1 at its argmax cell
+    if (hit) {
       double acc = 0.0;
       for (int b = lane; b < a.nb; b += 64) acc += a.dtp_next[b];
       acc = wave_sum(acc);
@@ -417,55 +461,61 @@ __global__ __launch_bounds__(ABLK) void k_adj_step(AdjStepArgs a) {
         }
       }
     }
-  }
-  const bool interior = inr && i > 0 && i < a.n - 1;
-
-  // primal of the cell: flux, acoustic tensor, u1; its CFL speed
-  double in[hes_gen::CELL_IN];
+    const bool interior = inr && i > 0 && i < a.n - 1;
+    const double dtdx = (a.cfl * a.dx / lmax_of(a.lmax, a.s)) / a.dx;
+    const double hdt = 0.5 * dtdx;
+    double in[hes_gen::CELL_IN];
 #pragma unroll
-  for (int r = 0; r < NQ; ++r) in[r] = q[r];
+    for (int r = 0; r < NQ; ++r) in[r] = q[r];
 #pragma unroll
-  for (int r = 0; r < 5; ++r) in[NQ + r] = a.gin[r];
-  double out[hes_gen::CELL_OUT];
-  hes_gen::cell_fwd(in, out);
-  double ob[hes_gen::CELL_OUT];
-  const double c = speed_and_grad(out + NQ, ob + NQ);
-  const double u1 = out[20];
-  const double lam = fabs(u1) + c;
-  const double lam_r = __shfl_down(lam, 1), lam_l = __shfl_up(lam, 1);
-  const double sR = fmax(lam, lam_r), sL = fmax(lam_l, lam);
-  const double dtdx = (a.cfl * a.dx / lmax_of(a.lmax, a.s)) / a.dx;
-  const double hdt = 0.5 * dtdx;
-
-  double sdR = 0.0, sdL = 0.0, fdot = 0.0;
+    for (int r = 0; r < 5; ++r) in[NQ + r] = a.gin[r];
+    double out[hes_gen::CELL_OUT];
+    double ib[hes_gen::CELL_IN];
 #pragma unroll
-  for (int r = 0; r < NQ; ++r) {
-    const double w = interior ? qb[r] : 0.0;
-    const double wr = __shfl_down(w, 1), wl = __shfl_up(w, 1);
-    const double qr = __shfl_down(q[r], 1), ql = __shfl_up(q[r], 1);
-    const double gR = wr - w, gL = w - wl;
-    const double gs = gL + gR;
-    ob[r] = hdt * gs;
-    fdot += out[r] * gs;
-    sdR += gR * (qr - q[r]);
-    sdL += gL * (q[r] - ql);
-    qb[r] += hdt * (sR * gR - sL * gL);
-  }
-  const double lbar = (lam >= lam_r ? -hdt * sdR : 0.0) + (lam > lam_l ? -hdt * sdL : 0.0);
+    for (int r = 0; r < hes_gen::CELL_IN; ++r) ib[r] = 0.0;
+    // seeds of the cell VJP from the primal (flux F, acoustic tensor, u1)
+    auto seed = [&](const double* o, double* ob) -> bool {
+      double gO[6];
+      const double c = speed_and_grad(o + NQ, gO);
+      const double u1 = o[20];
+      const double lam = fabs(u1) + c;
+      const double lam_r = __shfl_down(lam, 1), lam_l = __shfl_up(lam, 1);
+      const double sR = fmax(lam, lam_r), sL = fmax(lam_l, lam);
+      double sdR = 0.0, sdL = 0.0, fdot = 0.0;
 #pragma unroll
-  for (int r = NQ; r < NQ + 6; ++r) ob[r] *= lbar;
-  ob[20] = u1 < 0.0 ? -lbar : lbar;
-  const double dts = owned ? 0.5 * fdot - 0.5 * sR * sdR : 0.0;
-
-  double ib[hes_gen::CELL_IN];
+      for (int r = 0; r < NQ; ++r) {
+        const double w = interior ? qb[r] : 0.0;
+        const double wr = __shfl_down(w, 1), wl = __shfl_up(w, 1);
+        const double qr = __shfl_down(q[r], 1), ql = __shfl_up(q[r], 1);
+        const double gR = wr - w, gL = w - wl;
+        const double gs = gL + gR;
+        ob[r] = hdt * gs;
+        fdot += o[r] * gs;
+        sdR += gR * (qr - q[r]);
+        sdL += gL * (q[r] - ql);
+        qb[r] += hdt * (sR * gR - sL * gL);
+      }
+      const double lbar = (lam >= lam_r ? -hdt * sdR : 0.0) + (lam > lam_l ? -hdt * sdL : 0.0);
 #pragma unroll
-  for (int r = 0; r < hes_gen::CELL_IN; ++r) ib[r] = 0.0;
-  hes_gen::cell_vjp(in, ob, ib);
-  if (owned) {
+      for (int k = 0; k < 6; ++k) ob[NQ + k] = lbar * gO[k];
+      ob[20] = u1 < 0.0 ? -lbar : lbar;
+      dts = owned ? 0.5 * fdot - 0.5 * sR * sdR : 0.0;
+      return __any(lbar != 0.0);
+    };
+    hes_gen::cell_fwd_vjp(in, out, seed, ib);
+    bool zero = true;
+    if (owned) {
 #pragma unroll
-    for (int r = 0; r < NQ; ++r) a.Qb_out[r * pitch + i] = qb[r] + ib[r];
-    th0 += ib[NQ];
-    th1 += ib[NQ + 1];
+      for (int r = 0; r < NQ; ++r) {
+        const double v = qb[r] + ib[r];
+        a.Qb_out[r * pitch + i] = v;
+        zero = zero && v == 0.0;
+      }
+      th0 += ib[NQ];
+      th1 += ib[NQ + 1];
+    }
+    const int z = __all(zero);
+    if (lane == 0) a.zout[wave] = z;
   }
   const double bd = block_sum(dts);
   const double b0s = block_sum(th0);
@@ -475,6 +525,26 @@ __global__ __launch_bounds__(ABLK) void k_adj_step(AdjStepArgs a) {
     a.thb[2 * blockIdx.x] += b0s;
     a.thb[2 * blockIdx.x + 1] += c0s;
   }
+}
+
+// Zero flags of Qbar of the final state, per tile of TILE cells.
+__global__ __launch_bounds__(64) void k_adj_flags(const double* __restrict__ Qb,
+                                                  int* __restrict__ z, long long n,
+                                                  long long pitch, int nw) {
+  const long long w = blockIdx.x;
+  const long long i = w * TILE + threadIdx.x;
+  bool zero = true;
+  if (threadIdx.x < TILE && i < n) {
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) zero = zero && Qb[r * pitch + i] == 0.0;
+  }
+  const int all = __all(zero);
+  if (threadIdx.x == 0 && w < nw) z[w] = all;
+}
+
+__global__ void k_adj_izero(int* __restrict__ p, long long m) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) p[t] = 0;
 }
 
 // Per-block, per-piece sums of Qbar of state 0 (pieces are contiguous cell
@@ -574,6 +644,9 @@ struct hes_adjoint {
   long long n, pitch, nsteps, every, nck;
   int nb;    // blocks of the tiled kernels
   int nbl;   // blocks of the one-cell-per-thread kernels
+  int nw;    // tiles of TILE cells
+  int sparse = 1;
+  int use_graph = 1;  // replay captured graphs (else launch the kernels one by one)
   double cfl, dx;
   double mat8[8];
   MatDev mat;
@@ -583,6 +656,7 @@ struct hes_adjoint {
   double* fq[2] = {nullptr, nullptr};
   double* fc[2] = {nullptr, nullptr};
   double* qb[2] = {nullptr, nullptr};
+  int* zf = nullptr;                   // 2 x nw zero flags of qb[0], qb[1]
   unsigned long long* lmax = nullptr;  // nsteps + 1
   double* blk_val = nullptr;           // nsteps x nb
   long long* blk_idx = nullptr;
@@ -639,8 +713,10 @@ AdjFwdArgs fwd_args(hes_adjoint* h, long long s, const double* qi, double* qo, c
   a.Cin = ci;
   a.Cout = co;
   a.lmax = h->lmax;
-  a.blk_val = first_pass ? h->blk_val : nullptr;
-  a.blk_idx = first_pass ? h->blk_idx : nullptr;
+  // rows 1 .. nsteps-1 of the argmax record (the final state needs none)
+  const bool rec = first_pass && s + 1 < h->nsteps;
+  a.blk_val = rec ? h->blk_val : nullptr;
+  a.blk_idx = rec ? h->blk_idx : nullptr;
   a.status = h->status;
   a.n = h->n;
   a.pitch = h->pitch;
@@ -665,7 +741,7 @@ hipError_t record_forward(hes_adjoint* h) {
   hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((h->nsteps + 1 + 255) / 256)), dim3(256), 0, st,
                      (double*)h->lmax, h->nsteps + 1);
   hipLaunchKernelGGL(k_adj_speeds, dim3(nbl), dim3(LBLK), 0, st, ck_q(h, 0), ck_c(h, 0), n,
-                     h->pitch, h->mat, h->lmax, h->status);
+                     h->pitch, h->mat, h->lmax, h->blk_val, h->blk_idx, h->status);
   for (long long s = 0; s < h->nsteps; ++s) {
     double *qi, *ci, *qo, *co;
     fwd_slot(h, s, &qi, &ci);
@@ -674,7 +750,7 @@ hipError_t record_forward(hes_adjoint* h) {
                        fwd_args(h, s, qi, qo, ci, co, true));
   }
   hipLaunchKernelGGL(k_adj_argmax, dim3((unsigned)h->nsteps), dim3(64), 0, st, h->blk_val,
-                     h->blk_idx, h->nb, h->istar);
+                     h->blk_idx, h->nb, h->nbl, h->istar);
   double *qn, *cn;
   fwd_slot(h, h->nsteps, &qn, &cn);
   hipLaunchKernelGGL(k_adj_loss, dim3(nbl), dim3(LBLK), 0, st, qn, h->rho_t, h->u_t, h->mat8[0],
@@ -686,6 +762,12 @@ hipError_t record_backward(hes_adjoint* h) {
   hipStream_t st = h->stream;
   hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((2 * h->nb + 255) / 256)), dim3(256), 0, st,
                      h->thb, (long long)2 * h->nb);
+  // zero flags: those of Qbar_n from its values, the other buffer's unknown
+  const int fin = (int)(h->nsteps & 1);
+  hipLaunchKernelGGL(k_adj_flags, dim3((unsigned)h->nw), dim3(64), 0, st, h->qb[fin],
+                     h->zf + (long long)fin * h->nw, h->n, h->pitch, h->nw);
+  hipLaunchKernelGGL(k_adj_izero, dim3((unsigned)((h->nw + 255) / 256)), dim3(256), 0, st,
+                     h->zf + (long long)(1 - fin) * h->nw, (long long)h->nw);
   for (long long c = h->nck - 1; c >= 0; --c) {
     const long long s0 = c * h->every;
     const long long end = std::min(s0 + h->every, h->nsteps);
@@ -704,6 +786,8 @@ hipError_t record_backward(hes_adjoint* h) {
       a.Q = seg_slot(h, s, s0);
       a.Qb_in = h->qb[(s + 1) & 1];
       a.Qb_out = h->qb[s & 1];
+      a.zin = h->zf + ((s + 1) & 1) * h->nw;
+      a.zout = h->zf + (s & 1) * h->nw;
       a.lmax = h->lmax;
       a.istar = h->istar;
       a.Jlam = h->Jlam;
@@ -716,6 +800,8 @@ hipError_t record_backward(hes_adjoint* h) {
       a.cfl = h->cfl;
       a.dx = h->dx;
       a.nb = h->nb;
+      a.nw = h->nw;
+      a.sparse = h->sparse;
       a.has_next = s + 1 < h->nsteps ? 1 : 0;
       a.gin[0] = h->mat8[4];
       a.gin[1] = h->mat8[1];
@@ -773,6 +859,7 @@ int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* materi
   h->every = every < nsteps ? every : nsteps;
   h->nck = (nsteps + h->every - 1) / h->every;
   const long long nwaves = (ncells + TILE - 1) / TILE;
+  h->nw = (int)nwaves;
   h->nb = (int)((nwaves + ABLK / 64 - 1) / (ABLK / 64));
   h->nbl = (int)((ncells + LBLK - 1) / LBLK);
   h->cfl = cfl;
@@ -791,6 +878,7 @@ int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* materi
     if (e == hipSuccess) e = dalloc(&h->fc[b], P);
     if (e == hipSuccess) e = dalloc(&h->qb[b], 14 * P);
   }
+  if (e == hipSuccess) e = dalloc(&h->zf, (size_t)2 * h->nw);
   if (e == hipSuccess) e = dalloc(&h->lmax, (size_t)nsteps + 1);
   if (e == hipSuccess) e = dalloc(&h->blk_val, (size_t)nsteps * h->nb);
   if (e == hipSuccess) e = dalloc(&h->blk_idx, (size_t)nsteps * h->nb);
@@ -836,7 +924,7 @@ int hes_adjoint_destroy(hes_adjoint* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->gf) (void)hipGraphExecDestroy(h->gf);
   if (h->gb) (void)hipGraphExecDestroy(h->gb);
-  void* bufs[] = {h->ck, h->seg, h->fq[0], h->fq[1], h->fc[0], h->fc[1], h->qb[0], h->qb[1],
+  void* bufs[] = {h->ck, h->seg, h->fq[0], h->fq[1], h->fc[0], h->fc[1], h->qb[0], h->qb[1], h->zf,
                   h->lmax, h->blk_val, h->blk_idx, h->istar, h->Jlam, h->dtp, h->thb, h->lossp,
                   h->part, h->sums, h->gout, h->gin, h->prims, h->states, h->bounds, h->rho_t,
                   h->u_t, h->slots, h->status};
@@ -915,7 +1003,7 @@ int hes_adjoint_forward(hes_adjoint* h, double* loss, float* elapsed_ms) {
   if (!h) return fail(HES_ERR_ARG, "null handle");
   if (!h->has_init || !h->has_target) return fail(HES_ERR_ARG, "initial state and target must be set");
   HIP_TRY(zero_on(h->stream, h->status, 4 * sizeof(int)));
-  if (!h->gf) {
+  if (h->use_graph && !h->gf) {
     hipError_t e = capture(h, true, &h->gf);
     if (e != hipSuccess) {
       g_err = std::string("forward graph: ") + hipGetErrorString(e);
@@ -923,7 +1011,8 @@ int hes_adjoint_forward(hes_adjoint* h, double* loss, float* elapsed_ms) {
     }
   }
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
-  HIP_TRY(hipGraphLaunch(h->gf, h->stream));
+  if (h->use_graph) HIP_TRY(hipGraphLaunch(h->gf, h->stream));
+  else HIP_TRY(record_forward(h));
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipEventSynchronize(h->ev1));
   float ms = 0.f;
@@ -945,7 +1034,7 @@ int hes_adjoint_backward(hes_adjoint* h, double* grad_states, double* grad_mater
                          float* elapsed_ms) {
   if (!h) return fail(HES_ERR_ARG, "null handle");
   if (!h->fwd_done) return fail(HES_ERR_ARG, "run the forward pass first");
-  if (!h->gb) {
+  if (h->use_graph && !h->gb) {
     hipError_t e = capture(h, false, &h->gb);
     if (e != hipSuccess) {
       g_err = std::string("backward graph: ") + hipGetErrorString(e);
@@ -953,7 +1042,8 @@ int hes_adjoint_backward(hes_adjoint* h, double* grad_states, double* grad_mater
     }
   }
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
-  HIP_TRY(hipGraphLaunch(h->gb, h->stream));
+  if (h->use_graph) HIP_TRY(hipGraphLaunch(h->gb, h->stream));
+  else HIP_TRY(record_backward(h));
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipEventSynchronize(h->ev1));
   float ms = 0.f;
@@ -971,6 +1061,25 @@ int hes_adjoint_backward(hes_adjoint* h, double* grad_states, double* grad_mater
   return HES_OK;
 }
 
+int hes_adjoint_set_sparse(hes_adjoint* h, int on) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  if ((on != 0) != (h->sparse != 0)) {
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->gb) {
+      (void)hipGraphExecDestroy(h->gb);
+      h->gb = nullptr;
+    }
+    h->sparse = on != 0;
+  }
+  return HES_OK;
+}
+
+int hes_adjoint_set_launch(hes_adjoint* h, int use_graph) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  h->use_graph = use_graph != 0;
+  return HES_OK;
+}
+
 int hes_adjoint_get_state(hes_adjoint* h, double* Q) {
   if (!h || !Q) return fail(HES_ERR_ARG, "null argument");
   if (!h->fwd_done) return fail(HES_ERR_ARG, "run the forward pass first");
@@ -980,6 +1089,30 @@ int hes_adjoint_get_state(hes_adjoint* h, double* Q) {
   HIP_TRY(copy_on(h->stream, buf.data(), qn, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
   for (long long i = 0; i < h->n; ++i)
     for (int r = 0; r < 14; ++r) Q[i * 14 + r] = buf[r * h->pitch + i];
+  return HES_OK;
+}
+
+int hes_adjoint_get_profiles(hes_adjoint* h, double* rho, double* u1) {
+  if (!h || !rho || !u1) return fail(HES_ERR_ARG, "null argument");
+  if (!h->fwd_done) return fail(HES_ERR_ARG, "run the forward pass first");
+  double *qn, *cn;
+  fwd_slot(h, h->nsteps, &qn, &cn);
+  double *dr = nullptr, *du = nullptr;
+  HIP_TRY(dalloc(&dr, (size_t)h->n));
+  hipError_t e = dalloc(&du, (size_t)h->n);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_adj_profiles, dim3((unsigned)h->nbl), dim3(LBLK), 0, h->stream, qn,
+                       h->mat8[0], dr, du, h->n, h->pitch);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = copy_on(h->stream, rho, dr, sizeof(double) * h->n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = copy_on(h->stream, u1, du, sizeof(double) * h->n, hipMemcpyDeviceToHost);
+  (void)hipFree(dr);
+  if (du) (void)hipFree(du);
+  if (e != hipSuccess) {
+    g_err = std::string("get_profiles: ") + hipGetErrorString(e);
+    return HES_ERR_HIP;
+  }
   return HES_OK;
 }
 

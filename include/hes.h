@@ -153,8 +153,19 @@ int hes_adjoint_forward(hes_adjoint* h, double* loss, float* elapsed_ms);
  * layout as prims), grad_material[2] = (dL/db0, dL/dc0). */
 int hes_adjoint_backward(hes_adjoint* h, double* grad_states, double* grad_material,
                          float* elapsed_ms);
+/* on != 0 (the default): reverse steps skip tiles of cells whose cotangent is
+ * zero on the tile and its two halo cells (the result is the same bit for bit:
+ * such a tile's contribution is exactly zero); 0 evaluates every tile. */
+int hes_adjoint_set_sparse(hes_adjoint* h, int on);
+/* use_graph != 0 (the default): each pass is one captured hipGraph, replayed;
+ * 0: the kernels of a pass are launched one by one. */
+int hes_adjoint_set_launch(hes_adjoint* h, int use_graph);
 /* Final state of the forward run (14 x ncells, Julia layout). */
 int hes_adjoint_get_state(hes_adjoint* h, double* Q_host);
+/* rho and u1 of every cell of the final state, computed exactly as the misfit
+ * computes them (targets made from a run's profiles then give a zero misfit
+ * wherever the two runs agree bit for bit). */
+int hes_adjoint_get_profiles(hes_adjoint* h, double* rho, double* u1);
 /* Device memory held by the handle and the number of stored checkpoints. */
 int hes_adjoint_info(hes_adjoint* h, long long* device_bytes, long long* checkpoints);
 

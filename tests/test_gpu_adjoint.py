@@ -20,7 +20,7 @@ def _problem(n, steps, seed=7, every=None):
     """Two seeded pieces split at x = 1/2 and the target profiles of a run
     with perturbed shear and bulk speeds."""
     from hyperelasticsolver_amd.EquationsOfState import Barton2009
-    from hyperelasticsolver_amd.adjoint import Adjoint, final_profiles, phase_prims_to_adjoint
+    from hyperelasticsolver_amd.adjoint import Adjoint, phase_prims_to_adjoint
     from hyperelasticsolver_amd.initial import random_phase_prims
     rng = np.random.default_rng(seed)
     eos = Barton2009()
@@ -31,7 +31,7 @@ def _problem(n, steps, seed=7, every=None):
         t.set_initial(prims, [0.5])
         t.set_target(np.zeros(n), np.zeros(n))
         t.forward()
-        rho_t, u_t = final_profiles(eos_t, t.final_state())
+        rho_t, u_t = t.profiles()
     return eos, P14, prims, rho_t, u_t
 
 
@@ -44,14 +44,15 @@ def _loss(prims, b0, c0, n, steps, rho_t, u_t):
         return a.forward()
 
 
-def _gradient(eos, prims, rho_t, u_t, n, steps, every):
+def _gradient(eos, prims, rho_t, u_t, n, steps, every, sparse=True, interfaces=(0.5,)):
     from hyperelasticsolver_amd.adjoint import Adjoint
     with Adjoint(eos, n, steps, every=every) as a:
-        a.set_initial(prims, [0.5])
+        a.set_sparse(sparse)
+        a.set_initial(prims, list(interfaces))
         a.set_target(rho_t, u_t)
         L = a.forward()
         g, gm = a.backward()
-    return L, np.concatenate([g[:, 0], g[:, 1], gm])
+    return L, np.concatenate([g.ravel(order="F"), gm])
 
 
 def _params(prims, eos):
@@ -81,19 +82,84 @@ def test_adjoint_forward_is_the_solver_run(ora):
     assert np.all(rel_err_per_component(Qa, Qs) <= 1e-10)
 
 
-def test_checkpoint_interval_changes_nothing():
-    print("[adjoint] checkpoint interval invariance", flush=True)
-    n, steps = 400, 90
+def test_checkpoint_interval_and_tile_skipping_change_nothing():
+    print("[adjoint] checkpoint interval and zero-tile skipping invariance", flush=True)
+    n, steps = 4000, 90
     eos, _, prims, rho_t, u_t = _problem(n, steps, seed=5)
     ref = None
-    for every in (1, 7, 45, 90):
-        L, g = _gradient(eos, prims, rho_t, u_t, n, steps, every)
+    for every, sparse in ((1, True), (7, True), (45, False), (90, True), (90, False)):
+        L, g = _gradient(eos, prims, rho_t, u_t, n, steps, every, sparse=sparse)
         assert np.all(np.isfinite(g))
         if ref is None:
             ref = (L, g)
         else:
             assert L == ref[0]
-            assert np.array_equal(g, ref[1]), every
+            assert np.array_equal(g, ref[1]), (every, sparse)
+
+
+def test_many_pieces_sparse_equals_dense_and_matches_differences():
+    """Five pieces (four interfaces) on 2000 cells and a misfit observed on
+    cells 900..1099 only: zero-tile skipping on and off, graph replay and
+    one-by-one launches agree bit for bit; b0, c0 and one entry per piece
+    against central differences."""
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    from hyperelasticsolver_amd.adjoint import Adjoint, phase_prims_to_adjoint
+    from hyperelasticsolver_amd.initial import random_phase_prims
+    print("[adjoint] five pieces, sparse vs dense", flush=True)
+    n, steps, ifc = 2000, 300, [0.1, 0.35, 0.6, 0.62]
+    rng = np.random.default_rng(21)
+    eos = Barton2009()
+    prims = phase_prims_to_adjoint(random_phase_prims(rng, eos, 5))
+    eos_t = Barton2009(_b0=eos.b0 * SHEAR_UP, _c0=eos.c0 * BULK_DOWN)
+    with Adjoint(eos_t, n, steps, every=steps) as t:
+        t.set_initial(prims, ifc)
+        t.set_target(np.zeros(n), np.zeros(n))
+        t.forward()
+        rho_t, u_t = t.profiles()
+    # a local misfit: outside cells 900..1099 the target is the unperturbed run
+    with Adjoint(eos, n, steps, every=steps) as t:
+        t.set_initial(prims, ifc)
+        t.set_target(np.zeros(n), np.zeros(n))
+        t.forward()
+        rho_b, u_b = t.profiles()
+    out = np.r_[0:900, 1100:n]
+    rho_t[out] = rho_b[out]
+    u_t[out] = u_b[out]
+
+    def run(pr, b0, c0, sparse=True, grad=True, graph=True):
+        with Adjoint(Barton2009(_b0=b0, _c0=c0), n, steps, every=40) as a:
+            a.set_sparse(sparse)
+            a.set_launch(graph)
+            a.set_initial(pr, ifc)
+            a.set_target(rho_t, u_t)
+            L = a.forward()
+            if not grad:
+                return L
+            g, gm = a.backward()
+            return L, g, gm
+    L1, g1, m1 = run(prims, eos.b0, eos.c0, True)
+    L2, g2, m2 = run(prims, eos.b0, eos.c0, False)
+    L3, g3, m3 = run(prims, eos.b0, eos.c0, True, graph=False)
+    assert L1 == L2 and np.array_equal(g1, g2) and np.array_equal(m1, m2)
+    assert L1 == L3 and np.array_equal(g1, g3) and np.array_equal(m1, m3)
+    for which in ("b0", "c0"):
+        h = 1e-6 * getattr(eos, which)
+        kw = {"b0": eos.b0, "c0": eos.c0}
+        kp, km = dict(kw), dict(kw)
+        kp[which] += h
+        km[which] -= h
+        fd = (run(prims, kp["b0"], kp["c0"], grad=False) - run(prims, km["b0"], km["c0"], grad=False)) / (2 * h)
+        gv = m1[0 if which == "b0" else 1]
+        assert abs(gv - fd) <= 1e-6 * max(abs(fd), abs(gv)) + 1e-9 * np.abs(m1).max(), (which, gv, fd)
+    for j in range(5):
+        k = 3  # A11 of piece j
+        h = 1e-6 * abs(prims[k, j])
+        pp, pm = prims.copy(), prims.copy()
+        pp[k, j] += h
+        pm[k, j] -= h
+        fd = (run(pp, eos.b0, eos.c0, grad=False) - run(pm, eos.b0, eos.c0, grad=False)) / (2 * h)
+        scale = np.abs(g1).max()
+        assert abs(g1[k, j] - fd) <= 1e-6 * max(abs(fd), abs(g1[k, j])) + 1e-9 * scale, (j, g1[k, j], fd)
 
 
 def _check(g, fd, rel, what):

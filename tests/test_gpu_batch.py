@@ -155,6 +155,7 @@ def test_autotune_on_reused_memory_matches_clean_run(ora):
     runtime in the process, such as torch's, may not see the device once this
     library's runtime holds it)."""
     from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    from hyperelasticsolver_amd._lib import DomainError
     from hyperelasticsolver_amd.batched import Batch
     print("[batch] autotune on NaN-filled reused memory", flush=True)
     ncells = [300, 4000, 777, 5000] * 4
@@ -164,7 +165,8 @@ def test_autotune_on_reused_memory_matches_clean_run(ora):
     b.close()
     for n in (ncells, [sum(ncells) * 8]):
         p = Batch([Barton2009()] * len(n), n)
-        p.set_state(np.full((14, sum(n)), np.nan, order="F"))
+        with pytest.raises(DomainError):  # the state is stored, then rejected
+            p.set_state(np.full((14, sum(n)), np.nan, order="F"))
         p.close()
     b, *_ = _build(ora, ncells, seed=6)
     b.autotune(steps=3)
