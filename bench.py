@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FP64 cell-updates/s on 1 GPU (% HBM peak); batched vs plain; fwd+adj/fwd"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
+COUNTERS = os.path.join(ROOT, "profiles", "k_step_2p22_counters.json")
 BYTES_PER_CELL_1PH = 2 * 14 * 8  # read + write the 14-component FP64 state
 
 
@@ -130,7 +131,10 @@ def adjoint_problem(n, steps, every, seed=2):
     return eos, P14, prims, rho_t, u_t
 
 
-def time_adjoint(n=1 << 18, steps=2000, every=50):
+ADJOINT_MODES = ("graph_sparse", "graph_dense", "eager_sparse", "eager_dense")
+
+
+def time_adjoint(n=1 << 18, steps=2000, every=50, modes=ADJOINT_MODES):
     """(forward + adjoint) / forward: the plain forward is the solver's step
     loop on the same problem (one launch per step); the adjoint run is its
     checkpointed forward pass plus the reverse pass (recompute + reverse
@@ -151,29 +155,46 @@ def time_adjoint(n=1 << 18, steps=2000, every=50):
         a.set_initial(prims, [0.5])
         a.set_target(rho_t, u_t)
         info = a.info()
-        for graph in (True, False):
-            for sparse in (True, False):
-                a.set_launch(graph)
-                a.set_sparse(sparse)
-                a.forward()
-                a.backward()          # the first calls capture the graphs
-                loss = a.forward()
-                g, gm = a.backward()
-                grads.append(np.concatenate([g.ravel(), gm]))
-                runs[f"{'graph' if graph else 'eager'}_{'sparse' if sparse else 'dense'}"] = {
-                    "forward_ms": a.last_forward_ms, "backward_ms": a.last_backward_ms,
-                    "ratio": (a.last_forward_ms + a.last_backward_ms) / fwd_ms}
+        for mode in modes:
+            graph, sparse = mode.startswith("graph"), mode.endswith("sparse")
+            a.set_launch(graph)
+            a.set_sparse(sparse)
+            a.forward()
+            a.backward()          # the first calls capture the graphs
+            loss = a.forward()
+            g, gm = a.backward()
+            grads.append(np.concatenate([g.ravel(), gm]))
+            runs[mode] = {"forward_ms": a.last_forward_ms, "backward_ms": a.last_backward_ms,
+                          "ratio": (a.last_forward_ms + a.last_backward_ms) / fwd_ms}
     if not all(np.array_equal(grads[0], x) for x in grads[1:]):
         raise RuntimeError("launch modes disagree")
     if not np.all(np.isfinite(grads[0])):
         raise RuntimeError("non-finite gradient")
-    best = min((v["ratio"], k) for k, v in runs.items() if k.endswith("sparse"))[1]
-    dense = min((v["ratio"], k) for k, v in runs.items() if k.endswith("dense"))[1]
-    return {"cells": n, "steps": steps, "checkpoint_every": every, "loss": loss,
-            "forward_ms": fwd_ms, "fwd_plus_adj_over_fwd": runs[best]["ratio"], "mode": best,
-            "fwd_plus_adj_over_fwd_dense": runs[dense]["ratio"], "mode_dense": dense,
-            "runs": runs, "device_bytes": info["device_bytes"], "checkpoints": info["checkpoints"],
-            "grad_b0_c0": [float(gm[0]), float(gm[1])]}
+    out = {"cells": n, "steps": steps, "checkpoint_every": every, "loss": loss,
+           "forward_ms": fwd_ms, "runs": runs, "device_bytes": info["device_bytes"],
+           "checkpoints": info["checkpoints"], "grad_b0_c0": [float(gm[0]), float(gm[1])]}
+    for kind in ("sparse", "dense"):
+        cand = [(v["ratio"], k) for k, v in runs.items() if k.endswith(kind)]
+        if cand:
+            r, k = min(cand)
+            out["fwd_plus_adj_over_fwd" + ("" if kind == "sparse" else "_dense")] = r
+            out["mode" + ("" if kind == "sparse" else "_dense")] = k
+    return out
+
+
+def measured_traffic(nx, flux):
+    """HBM bytes per launch of the step kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (calibrated on a copy kernel), when they were
+    taken on this workload; else None."""
+    try:
+        with open(COUNTERS) as f:
+            c = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = c.get(flux)
+    if not rec or rec.get("cells") != nx:
+        return None
+    return rec["traffic_bytes"]
 
 
 def main():
@@ -184,7 +205,11 @@ def main():
     ap.add_argument("--nx", type=int, default=1 << 22)
     ap.add_argument("--flux", default="rusanov", choices=["rusanov", "hll"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the HLL and batched lines")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the headline line (no HLL, batched or adjoint lines)")
+    ap.add_argument("--parts", default="hll,batched,adjoint",
+                    help="extra lines to add: any of hll, batched, adjoint")
+    ap.add_argument("--adjoint-modes", default=",".join(ADJOINT_MODES))
     ap.add_argument("--batched-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -200,6 +225,8 @@ def main():
     ms_step = ms / args.steps
     value = nx * args.steps / (ms / 1e3)
     achieved = BYTES_PER_CELL_1PH * nx / (ms_step / 1e3) / 1e9
+    from hyperelasticsolver_amd.solver import copy_bandwidth
+    copy_gbs = copy_bandwidth(1 << 30, 5)
     out = {
         "metric": METRIC,
         "value": value,
@@ -217,19 +244,23 @@ def main():
                                f"{args.flux}, CFL 0.6, Barton2009 copper",
                    "cells": nx, "flux": args.flux, "interfaces": 64},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_step<1,{0 if args.flux == 'rusanov' else 1}>",
-                     "bytes_per_cell_update": BYTES_PER_CELL_1PH},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(nx, args.flux),
+                     "kernel": f"k_step<1,{0 if args.flux == 'rusanov' else 1},1>",
+                     "bytes_per_cell_update": BYTES_PER_CELL_1PH,
+                     "copy_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "device": device_info(),
     }
-    if not args.no_extra:
+    parts = set() if args.no_extra else set(p for p in args.parts.split(",") if p)
+    if "hll" in parts:
         other = "hll" if args.flux == "rusanov" else "rusanov"
         ms2 = time_solver(eos, Qs, ifc, nx, other, args.steps, args.warmup)
         v2 = nx * args.steps / (ms2 / 1e3)
         out[other] = {"value": v2, "ms_per_step": ms2 / args.steps,
                       "hbm_frac": BYTES_PER_CELL_1PH * v2 / 1e9 / HBM_PEAK_GBS}
+    if "batched" in parts:
         out["batched"] = time_batched(args.batched_steps)
-        out["adjoint"] = time_adjoint()
+    if "adjoint" in parts:
+        out["adjoint"] = time_adjoint(modes=tuple(m for m in args.adjoint_modes.split(",") if m))
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
         out["cpu_baseline"] = cpu_baseline(eos, Qs, ifc, 0 if args.flux == "rusanov" else 1, threads)

@@ -414,7 +414,12 @@ struct AdjStepArgs {
 // cells and both halo cells (and which holds no argmax cell of step s+1) has
 // Qbar of state s = 0 and contributes nothing to dtdxbar or (b0bar, c0bar):
 // every seed of its cell VJPs is zero, and the VJP is linear in its seeds.
-__global__ __launch_bounds__(ABLK) void k_adj_step(AdjStepArgs a) {
+#ifndef HES_ADJ_WAVES_PER_EU
+#define HES_ADJ_WAVES_PER_EU 1
+#endif
+
+__global__ __launch_bounds__(ABLK) __attribute__((amdgpu_waves_per_eu(HES_ADJ_WAVES_PER_EU)))
+void k_adj_step(AdjStepArgs a) {
   const int lane = threadIdx.x & 63;
   const long long wave = (xcd_block() * ABLK + threadIdx.x) >> 6;
   const long long i = wave * TILE + lane - 1;

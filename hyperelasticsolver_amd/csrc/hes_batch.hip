@@ -20,13 +20,19 @@
 
 using namespace hes;
 
+// One tile: its first owned cell and its problem's cell range, in one record
+// so a wave's state loads depend on a single (scalar) load.
+struct TileDesc {
+  long long first, lo, hi;
+  int p, pad;
+};
+
 struct BatchArgs {
   const double* __restrict__ Qin;
   double* __restrict__ Qout;
   const double* __restrict__ Cin;
   double* __restrict__ Cout;
-  const long long* __restrict__ tile_start;  // first owned cell of each tile
-  const int* __restrict__ tile_prob;         // problem of each tile
+  const TileDesc* __restrict__ tiles;
   const long long* __restrict__ offsets;     // P + 1
   const MatDev* __restrict__ mats;           // nphase per problem
   const double* __restrict__ cfl;
@@ -64,22 +70,22 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? 3 : 1))) void k_batch_step(BatchAr
   const int lane = threadIdx.x & 63;
   const int tile = batch_tile<BLK>(a);
   if (tile >= a.tile_end) return;  // the whole wave (no block-level sync here)
-  const int p = a.tile_prob[tile];
-  const long long lo = a.offsets[p], hi = a.offsets[p + 1];
-  const long long first = a.tile_start[tile];
+  const TileDesc td = a.tiles[tile];
+  const int p = td.p;
+  const long long lo = td.lo, hi = td.hi, first = td.first;
   const bool lead = first == lo;  // this tile does the problem's bookkeeping
+  // the problem's materials are read in place (wave-uniform loads); a local
+  // copy whose address is passed on would live in scratch memory
+  const MatDev* m = a.mats + NPH * p;
+  const ChunkCtx ctx = {m, a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, lo, hi, a.cfl[p], a.dx[p]};
+  const long long i = first + lane - 1;
+  double q[NC], cc[NPH];
+  load_chunk<NPH>(ctx, i, q, cc);  // in flight while the problem's time is read
   const double tcur = a.tring[2 * p + (a.step & 1)];
   if (!(tcur < a.t_end[p])) {
     if (lead && lane == 0) a.tring[2 * p + ((a.step + 1) & 1)] = tcur;
     return;
   }
-  MatDev m[NPH];
-#pragma unroll
-  for (int k = 0; k < NPH; ++k) m[k] = a.mats[NPH * p + k];
-  const ChunkCtx ctx = {m, a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, lo, hi, a.cfl[p], a.dx[p]};
-  const long long i = first + lane - 1;
-  double q[NC], cc[NPH];
-  load_chunk<NPH>(ctx, i, q, cc);
   const double lmax = __longlong_as_double(*(const long long*)(a.slots + 3 * p + (a.step % 3)));
   double lam = chunk_step<NPH, FLUX, FAST>(ctx, i, q, cc, lmax);
   lam = wave_max(lam);
@@ -101,12 +107,11 @@ __global__ __launch_bounds__(64) void k_batch_speeds(BatchArgs a) {
   const int lane = threadIdx.x & 63;
   const int tile = __builtin_amdgcn_readfirstlane(a.tile_begin + (int)blockIdx.x);
   if (tile >= a.tile_end) return;
-  const int p = a.tile_prob[tile];
-  const long long lo = a.offsets[p], hi = a.offsets[p + 1];
-  const long long i = a.tile_start[tile] + lane - 1;
-  MatDev m[NPH];
-#pragma unroll
-  for (int k = 0; k < NPH; ++k) m[k] = a.mats[NPH * p + k];
+  const TileDesc td = a.tiles[tile];
+  const int p = td.p;
+  const long long lo = td.lo, hi = td.hi;
+  const long long i = td.first + lane - 1;
+  const MatDev* m = a.mats + NPH * p;
   double lam = 0.0;
   int bad = 0;
   if (lane >= 1 && lane <= TILE && i < hi) {
@@ -188,8 +193,7 @@ struct hes_batch {
   std::vector<long long> offsets_h;
   std::vector<int> tile_first_h;  // first tile of each problem (P + 1)
   double *Q[2], *C[2];
-  long long* tile_start;
-  int* tile_prob;
+  TileDesc* tiles;
   long long* offsets;
   MatDev* mats;
   double *cfl, *dx, *t_end, *tring;
@@ -209,8 +213,7 @@ BatchArgs batch_args(hes_batch* h, long long s, int tb, int te) {
   a.Qout = h->Q[(s + 1) & 1];
   a.Cin = h->C[s & 1];
   a.Cout = h->C[(s + 1) & 1];
-  a.tile_start = h->tile_start;
-  a.tile_prob = h->tile_prob;
+  a.tiles = h->tiles;
   a.offsets = h->offsets;
   a.mats = h->mats;
   a.cfl = h->cfl;
@@ -295,7 +298,7 @@ extern "C" {
 int hes_batch_destroy(hes_batch* h) {
   if (!h) return HES_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  void* ptrs[] = {h->Q[0], h->Q[1], h->C[0], h->C[1], h->tile_start, h->tile_prob, h->offsets,
+  void* ptrs[] = {h->Q[0], h->Q[1], h->C[0], h->C[1], h->tiles, h->offsets,
                   h->mats, h->cfl, h->dx, h->t_end, h->tring, h->slots, h->psteps, h->status};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -360,8 +363,10 @@ int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const
     if (e == hipSuccess) e = hipMalloc(d, bytes);
     if (e == hipSuccess && src) e = copy_on(h->stream, *d, src, bytes, hipMemcpyHostToDevice);
   };
-  up((void**)&h->tile_start, tstart.data(), sizeof(long long) * tstart.size());
-  up((void**)&h->tile_prob, tprob.data(), sizeof(int) * tprob.size());
+  std::vector<TileDesc> tdesc(tstart.size());
+  for (size_t t = 0; t < tstart.size(); ++t)
+    tdesc[t] = {tstart[t], h->offsets_h[tprob[t]], h->offsets_h[tprob[t] + 1], tprob[t], 0};
+  up((void**)&h->tiles, tdesc.data(), sizeof(TileDesc) * tdesc.size());
   up((void**)&h->offsets, h->offsets_h.data(), sizeof(long long) * (nproblems + 1));
   up((void**)&h->mats, mats.data(), sizeof(MatDev) * mats.size());
   up((void**)&h->cfl, vcfl.data(), sizeof(double) * nproblems);

@@ -1,0 +1,210 @@
+"""The module API (Strains, EquationsOfState, HyperelasticityMPh, NumFluxes and
+the repaired single-phase Hyperelasticity) on the GPU against the oracle, on
+seeded states: A = I + 0.05 N(0,1), u ~ U(-1/2, 1/2)^3, S ~ U(0, 2e-3)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MAT1 = np.array([8.93, 4.6, 3.9e-4, 300.0, 2.1, 1.0, 3.0, 2.0])
+MAT2 = np.array([2.7, 4.6, 3.9e-4, 300.0, 3.1, 1.0, 3.0, 2.0])
+NPTS = 48
+
+
+def _eos():
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    return Barton2009(), Barton2009(_rho0=2.7, _b0=3.1)
+
+
+def _close(a, b, rtol, what=""):
+    """|a - b| <= rtol |b| + 1e-14 max(|b|, 1): relative, with a floor for
+    entries that vanish by cancellation (e.g. the energy of the state at rest)."""
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    tol = rtol * np.abs(b) + 1e-14 * max(np.abs(b).max(), 1.0)
+    excess = (np.abs(a - b) - tol).max()
+    assert excess <= 0.0, (what, np.abs(a - b).max(), excess)
+
+
+def _A(rng):
+    A = np.eye(3) + 0.05 * rng.standard_normal((3, 3))
+    return A.reshape(-1, order="F")
+
+
+def _phase_prims(rng, mat):
+    A = _A(rng)
+    return np.concatenate([[mat[0] / np.linalg.det(A.reshape(3, 3, order="F"))],
+                           rng.uniform(-0.5, 0.5, 3), [rng.uniform(0, 2e-3)], A])
+
+
+def _two_phase(rng):
+    a = rng.uniform(0.01, 0.99)
+    p1 = _phase_prims(rng, MAT1)
+    p2 = _phase_prims(rng, MAT2)
+    return np.concatenate([[a], p1, [1.0 - a], p2])
+
+
+def test_strains(ora):
+    from hyperelasticsolver_amd.Strains import finger, invariants
+    print("[pointwise] Strains", flush=True)
+    rng = np.random.default_rng(1)
+    A = np.stack([_A(rng) for _ in range(NPTS)], axis=1)
+    G = finger(A)
+    for k in range(NPTS):
+        _close(G[:, k], ora.finger(A[:, k]), 1e-13, "finger")
+        _close(invariants(G[:, k]), ora.invariants(G[:, k]), 1e-13, "invariants")
+
+
+def test_equations_of_state(ora):
+    from hyperelasticsolver_amd import EquationsOfState as E
+    from hyperelasticsolver_amd.Strains import finger
+    print("[pointwise] EquationsOfState", flush=True)
+    eos = _eos()[0]
+    rng = np.random.default_rng(2)
+    for _ in range(NPTS):
+        A = _A(rng)
+        S = rng.uniform(0, 2e-3)
+        G = finger(A)
+        e = E.energy(eos, S, G)
+        assert e == pytest.approx(ora.energy(MAT1, S, G), rel=1e-13)
+        assert E.entropy(eos, e, G) == pytest.approx(ora.entropy(MAT1, e, G), rel=1e-10, abs=1e-15)
+        assert E.temperature(eos, S, G) == pytest.approx(ora.temperature(MAT1, S, G), rel=1e-13)
+        _close(E.stress(eos, S, A), ora.stress(MAT1, S, A), 1e-11, "stress")
+        den = MAT1[0] / np.linalg.det(A.reshape(3, 3, order="F"))
+        _close(E.stress(eos, den, e, A), ora.stress_den(MAT1, den, e, A), 1e-10, "stress_den")
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        _close(E.acoustic(eos, S, A, n), ora.acoustic(MAT1, S, A, n), 1e-10, "acoustic")
+        Pv = np.concatenate([[1.0, den], rng.uniform(-0.5, 0.5, 3), [S], A])
+        _close(E.acoustic(eos, Pv, n), ora.acoustic_prim(MAT1, Pv, n), 1e-10, "acoustic_prim")
+    # the entropy clamp (EquationsOfState.jl:152-154): far below the cold energy
+    G = finger(np.eye(3).reshape(-1) * 0.9)
+    assert E.entropy(eos, -10.0, G) == pytest.approx(ora.entropy(MAT1, -10.0, G), rel=1e-13)
+
+
+def test_multiphase_module(ora):
+    from hyperelasticsolver_amd import HyperelasticityMPh as H
+    print("[pointwise] HyperelasticityMPh", flush=True)
+    eos = _eos()
+    mats = np.concatenate([MAT1, MAT2])
+    rng = np.random.default_rng(3)
+    P = np.stack([_two_phase(rng) for _ in range(NPTS)], axis=1)
+    Q = H.prim2cons_mph(eos, P)
+    _close(Q, ora.prim2cons(2, mats, P), 1e-13, "prim2cons_mph")
+    _close(H.cons2prim_mph(eos, Q), ora.cons2prim(2, mats, Q), 1e-10, "cons2prim_mph")
+    _close(H.flux_mph(eos, Q), ora.flux(2, mats, Q), 1e-11, "flux_mph")
+    ev = H.get_eigvals(eos, Q, [1.0, 0.0, 0.0])
+    eo = ora.get_eigvals(2, mats, Q)
+    for k in range(NPTS):
+        _close(np.sort(ev[:, k]), np.sort(eo[:, k]), 1e-10, "get_eigvals")
+    for k in range(0, NPTS, 8):
+        _close(H.noncons_flux(eos, Q[:, k]), ora.noncons_flux(mats, Q[:, k]), 1e-10, "noncons")
+        # per-phase methods on a 15-component block
+        blk = Q[:15, k]
+        _close(H.flux(eos[0], blk), ora.flux(2, mats, Q[:, k])[:15], 1e-11, "phase flux")
+        _close(H.cons2prim(eos[0], blk), ora.cons2prim(2, mats, Q[:, k])[:15], 1e-10, "phase c2p")
+
+
+def test_multiphase_initial_states(ora):
+    from hyperelasticsolver_amd import HyperelasticityMPh as H
+    print("[pointwise] HyperelasticityMPh.initial_states", flush=True)
+    eos = _eos()
+    mats = np.concatenate([MAT1, MAT2])
+    for tc in (1, 2, 3, 4, 5, 6, 7, 10):
+        Pl, Pr = H.initial_prims(tc)
+        Ql, Qr = H.initial_states(eos, tc)
+        _close(Ql, ora.prim2cons(2, mats, Pl), 1e-13, ("Ql", tc))
+        _close(Qr, ora.prim2cons(2, mats, Pr), 1e-13, ("Qr", tc))
+
+
+@pytest.mark.parametrize("nph", [1, 2])
+def test_numerical_fluxes(ora, nph):
+    from hyperelasticsolver_amd import NumFluxes
+    print(f"[pointwise] NumFluxes nph={nph}", flush=True)
+    rng = np.random.default_rng(4 + nph)
+    if nph == 2:
+        eos, mats = _eos(), np.concatenate([MAT1, MAT2])
+        mk = lambda: _two_phase(rng)
+    else:
+        eos, mats = _eos()[0], MAT1
+        mk = lambda: _phase_prims(rng, MAT1)
+    for _ in range(8):
+        Ql = ora.prim2cons(nph, mats, mk())
+        Qr = ora.prim2cons(nph, mats, mk())
+        lam = rng.uniform(3.0, 6.0)
+        got = NumFluxes.lxf(eos, Ql, Qr, lam)
+        ref = ora.lxf(nph, mats, Ql, Qr, lam)
+        for g, r, nm in zip(got, ref, ("cons", "dm", "dp")):
+            _close(g, r, 1e-10, ("lxf", nm))
+        el = ora.get_eigvals(nph, mats, Ql)
+        er = ora.get_eigvals(nph, mats, Qr)
+        got = NumFluxes.hll(eos, Ql, Qr, (el, er))
+        ref = ora.hll(nph, mats, Ql, Qr, el, er)
+        for g, r, nm in zip(got, ref, ("cons", "dm", "dp")):
+            _close(g, r, 1e-9, ("hll", nm))
+
+
+def _to14(q13, rho):
+    q = np.zeros(14)
+    q[0] = rho
+    q[1:4] = q13[0:3]
+    q[4] = q13[12]
+    F = q13[3:12].reshape(3, 3)          # row-major rho F
+    q[5:14] = F.reshape(-1, order="F")
+    return q
+
+
+def test_single_phase_module(ora):
+    from hyperelasticsolver_amd import Hyperelasticity as H
+    from hyperelasticsolver_amd.EquationsOfState import energy
+    from hyperelasticsolver_amd.Strains import finger
+    print("[pointwise] Hyperelasticity (single phase)", flush=True)
+    eos = _eos()[0]
+    rng = np.random.default_rng(6)
+    for tc in (1, 2, 3, 4):
+        Ql, Qr = H.initial_states(eos, tc)
+        assert Ql.shape == (13,) and Qr.shape == (13,)
+    for _ in range(16):
+        F = np.eye(3) + 0.05 * rng.standard_normal((3, 3))
+        vel = rng.uniform(-0.5, 0.5, 3)
+        S = rng.uniform(0, 2e-3)
+        Q = H.prim2cons(eos, vel, F, S)
+        den = MAT1[0] / np.linalg.det(F)
+        # layout of Hyperelasticity.jl:77-81 (row-major rho F) and :90-94
+        assert Q[0:3] == pytest.approx(den * vel, rel=1e-13)
+        assert Q[3:12] == pytest.approx(den * F.reshape(-1), rel=1e-13)
+        e_int = energy(eos, S, finger(F.reshape(-1, order="F")))
+        assert Q[12] == pytest.approx(den * (e_int + 0.5 * vel @ vel), rel=1e-13)
+        d2, v2, F2, e2 = H.cons2prim(eos, Q)
+        assert d2 == pytest.approx(den, rel=1e-13)
+        assert v2 == pytest.approx(vel, rel=1e-12, abs=1e-15)
+        assert F2 == pytest.approx(F, rel=1e-13)
+        assert e2 == pytest.approx(e_int, rel=1e-9, abs=1e-14)
+        # flux: the one-phase physics in the 13-component order
+        f = H.flux(eos, Q)
+        f14 = ora.flux(1, MAT1, _to14(Q, den))
+        assert f[0:3] == pytest.approx(f14[1:4], rel=1e-11, abs=1e-12)
+        assert f[12] == pytest.approx(f14[4], rel=1e-11, abs=1e-12)
+        assert f[3:6] == pytest.approx(0.0, abs=1e-12)
+        assert f[3:12] == pytest.approx(f14[5:14].reshape(3, 3, order="F").reshape(-1), rel=1e-11, abs=1e-12)
+    Q0 = np.stack([H.prim2cons(eos, rng.uniform(-0.5, 0.5, 3), np.eye(3) + 0.05 * rng.standard_normal((3, 3)),
+                               1e-3) for _ in range(10)], axis=1)
+    den, ent, vel, strs, eint, info = H.postproc_arrays(eos, Q0)
+    assert info == ("den", "ent", "vel", "strs", "eint", "info")
+    assert ent == pytest.approx(np.full(10, 1e-3), rel=1e-8)
+    for i in range(10):
+        F = Q0[3:12, i].reshape(3, 3) / den[i]
+        sig = ora.stress_den(MAT1, den[i], eint[i], F.reshape(-1, order="F")).reshape(3, 3, order="F")
+        assert strs[:, :, i] == pytest.approx(sig.T, rel=1e-10, abs=1e-12)
+
+
+def test_domain_error_on_inverted_state():
+    from hyperelasticsolver_amd import HyperelasticityMPh as H
+    from hyperelasticsolver_amd._lib import DomainError
+    print("[pointwise] DomainError", flush=True)
+    eos = _eos()
+    P = _two_phase(np.random.default_rng(7))
+    Q = H.prim2cons_mph(eos, P)
+    Q[6:15] *= -1.0   # det(rho A) < 0 in phase 1 (sqrt of a negative, HyperelasticityMPh.jl:114)
+    with pytest.raises(DomainError):
+        H.cons2prim_mph(eos, Q)
