@@ -50,6 +50,7 @@ struct StepArgs {
   int* status;                // [0] error bits, [1] executed steps
   long long n, pitch;
   double cfl, dx, t_end;
+  double relax;               // two phases: 1 / tau of the velocity relaxation (0: none)
   int step;                   // step index modulo 6 (ring phase)
   int hist_idx;
   MatDev mat[2];
@@ -118,7 +119,8 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : 1))) void 
   const long long i = wave * TILE + lane - 1;
   double tcur;
   if (!step_begin(a, tcur)) return;
-  const ChunkCtx ctx = {a.mat, a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, 0, a.n, a.cfl, a.dx};
+  const ChunkCtx ctx = {a.mat, a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, 0, a.n, a.cfl, a.dx,
+                        a.relax};
   double q[NC], cc[NPH];
   load_chunk<NPH>(ctx, i, q, cc);
   const double lmax = step_lmax(a);
@@ -414,6 +416,7 @@ struct hes_solver {
   double mats[16];
   MatDev mat[2];
   double cfl, dx;
+  double relax;         // 1 / tau of the two-phase velocity relaxation (0: none)
   double* Q[2];
   double* C[2];
   unsigned long long* slots;
@@ -470,6 +473,7 @@ StepArgs make_args(hes_solver* h, long long s, double t_end, double* hist, int h
   a.cfl = h->cfl;
   a.dx = h->dx;
   a.t_end = t_end;
+  a.relax = h->relax;
   a.step = (int)(s % 6);
   a.hist_idx = hist_idx;
   a.mat[0] = h->mat[0];
@@ -572,6 +576,14 @@ int hes_solver_create(hes_solver** out, int nphase, int flux, long long ncells,
     return HES_ERR_HIP;
   }
   *out = h;
+  return HES_OK;
+}
+
+int hes_solver_set_relaxation(hes_solver* h, double tau) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  if (h->nph != 2) return fail(HES_ERR_ARG, "velocity relaxation needs two phases");
+  if (!(tau > 0.0)) return fail(HES_ERR_ARG, "tau must be positive (infinity: off)");
+  h->relax = std::isinf(tau) ? 0.0 : 1.0 / tau;
   return HES_OK;
 }
 

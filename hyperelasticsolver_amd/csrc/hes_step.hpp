@@ -68,7 +68,45 @@ struct ChunkCtx {
   int* status;
   long long pitch, lo, hi;
   double cfl, dx;
+  double relax = 0.0;  // two phases: 1 / tau of the interphase velocity relaxation (0: none)
 };
+
+// Interphase velocity relaxation of a two-phase cell over a time dt (an
+// optional source; the reference has none): the drag d(m1 u1)/dt = mu (u2 - u1)
+// / tau, mu = m1 m2 / (m1 + m2), with m_p = a_p rho_p as cons2prim gives it.
+// Solved exactly: the mixture velocity stays, the relative velocity decays by
+// exp(-dt / tau).  The kinetic energy it dissipates goes to the two phases in
+// equal halves, so mass, momentum and total energy are unchanged.
+__device__ __forceinline__ void relax_velocity2(const MatDev* mat, double* q, double decay) {
+  double m[2], u[2][3];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const double* b = q + 15 * p;
+    const double ia = rcp(b[0]);
+    m[p] = b[0] * sqrt(det3cm(b + 6) * (ia * ia * ia) * mat[p].irho0);
+    const double im = rcp(m[p]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) u[p][k] = b[2 + k] * im;
+  }
+  const double M = m[0] + m[1], iM = rcp(M);
+  const double mu = m[0] * m[1] * iM;
+  double dk = 0.0, ke[2] = {0.0, 0.0}, kn[2] = {0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double U = (m[0] * u[0][k] + m[1] * u[1][k]) * iM;
+    const double ur = u[1][k] - u[0][k], urn = ur * decay;
+    const double v0 = U - m[1] * iM * urn, v1 = U + m[0] * iM * urn;
+    dk += 0.5 * mu * (ur * ur - urn * urn);
+    ke[0] += u[0][k] * u[0][k];
+    ke[1] += u[1][k] * u[1][k];
+    kn[0] += v0 * v0;
+    kn[1] += v1 * v1;
+    q[2 + k] = m[0] * v0;
+    q[17 + k] = m[1] * v1;
+  }
+  q[5] += 0.5 * m[0] * (kn[0] - ke[0]) + 0.5 * dk;
+  q[20] += 0.5 * m[1] * (kn[1] - ke[1]) + 0.5 * dk;
+}
 
 template <int NPH, int FLUX, int FAST>
 __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, const double* q,
@@ -142,6 +180,8 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
 
   const bool owned = inr && lane >= 1 && lane <= TILE;
   const bool interior = owned && i > a.lo && i < a.hi - 1;
+  if (NPH == 2 && a.relax > 0.0 && interior)
+    relax_velocity2(a.mat, upd, exp(-(a.cfl * a.dx / lmax) * a.relax));
   double lam_new = 0.0;
   int bad2 = 0;
   if (interior) {

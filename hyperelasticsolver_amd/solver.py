@@ -129,6 +129,12 @@ class Solver:
         check(lib().hes_solver_next_dt(self._h, ctypes.byref(d)))
         return d.value
 
+    def set_relaxation(self, tau=math.inf):
+        """Two phases: interphase velocity relaxation with time scale tau after
+        every flux update (an optional source; the reference has none).
+        tau = inf switches it off."""
+        check(lib().hes_solver_set_relaxation(self._h, float(tau)))
+
     # -- stepping
     def run(self, steps=None, t_end=math.inf, use_graph=True, record_dt=False):
         """Advance up to `steps` steps while t < t_end.  Returns a dict with the

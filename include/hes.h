@@ -80,6 +80,14 @@ int hes_solver_next_dt(hes_solver* h, double* dt);
 int hes_solver_run(hes_solver* h, long long max_steps, double t_end, int use_graph,
                    long long* steps_done, double* t_now, float* elapsed_ms, double* dt_out);
 
+/* Optional source, two phases only (the reference has no source terms):
+ * interphase velocity relaxation with time scale tau, applied to the interior
+ * cells after each flux update.  The relative velocity decays exactly by
+ * exp(-dt / tau); mixture momentum, phase masses and total energy are kept (the
+ * dissipated kinetic energy goes to the two phases in equal halves).
+ * tau = INFINITY switches it off (the default). */
+int hes_solver_set_relaxation(hes_solver* h, double tau);
+
 /* Time `reps` back-to-back launches of the step kernel (and nothing else) with
  * HIP events on the solver stream; the state advances by reps steps. */
 int hes_solver_time_steps(hes_solver* h, long long reps, float* elapsed_ms);

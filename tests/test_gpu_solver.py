@@ -188,3 +188,74 @@ def test_golden_testcase6():
     err = rel_err_per_component(Q, g["Q"])
     print(f"  max per-component rel err {err.max():.3e}", flush=True)
     assert np.all(err <= TOL), err
+
+
+def _two_phase_eos(mats):
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    mk = lambda m: Barton2009(_rho0=m[0], _c0=m[1], _cv=m[2], _t0=m[3], _b0=m[4], _alpha=m[5],
+                              _beta=m[6], _gamma=m[7])
+    return mk(mats[:8]), mk(mats[8:])
+
+
+def test_relaxation_decays_relative_velocity_exactly(ora):
+    """Velocity relaxation (an optional source; parity unpinned, the reference
+    has none).  On a uniform state the fluxes cancel exactly, so only the
+    source acts: the relative velocity decays as exp(-t / tau), and mixture
+    momentum and total energy stay to round-off."""
+    from hyperelasticsolver_amd.solver import Solver
+    print("[relaxation] uniform two-phase state", flush=True)
+    mats, _, Qs, _ = _problem(ora, 2, 10, 1, seed=8)
+    q = Qs[:, 0]
+    nx, tau, steps = 200, 2e-3, 40
+    Q0 = np.asfortranarray(np.repeat(Qs[:, :1], nx, axis=1))
+    with Solver(_two_phase_eos(mats), nx, flux="rusanov", cfl=0.6) as s:
+        s.set_relaxation(tau)
+        s.set_state(Q0)
+        r = s.run(steps=steps, use_graph=False)
+        Q = s.get_state()
+    assert r["steps"] == steps
+    P0 = ora.cons2prim(2, mats, q)
+    P1 = ora.cons2prim(2, mats, Q[:, nx // 2])
+    ur0 = P0[17:20] - P0[2:5]
+    ur1 = P1[17:20] - P1[2:5]
+    np.testing.assert_allclose(ur1, ur0 * math.exp(-r["t"] / tau), rtol=1e-11, atol=1e-14)
+    mom0 = q[2:5] + q[17:20]
+    mom1 = Q[2:5, nx // 2] + Q[17:20, nx // 2]
+    np.testing.assert_allclose(mom1, mom0, rtol=1e-13, atol=1e-13 * np.abs(mom0).max())
+    assert Q[5, nx // 2] + Q[20, nx // 2] == pytest.approx(q[5] + q[20], rel=1e-13)
+    # masses and volume fractions are untouched away from the frozen end cells
+    # (which do not relax, so a disturbance enters from each end at one cell
+    # per step); the end cells themselves never change
+    inner = slice(steps + 1, nx - steps - 1)
+    assert np.array_equal(Q[[0, 1, 15, 16], inner], Q0[[0, 1, 15, 16], inner])
+    assert np.array_equal(Q[:, [0, nx - 1]], Q0[:, [0, nx - 1]])
+
+
+def test_relaxation_off_is_the_plain_step_and_conserves(ora):
+    """tau = inf gives the source-free step bit for bit; with a finite tau a
+    two-phase Riemann run keeps mixture momentum and total energy telescoping."""
+    from hyperelasticsolver_amd.solver import Solver
+    print("[relaxation] off == plain; conservation with the source", flush=True)
+    rng = np.random.default_rng(9)
+    mats = np.concatenate([MAT_CU, MAT_AL])
+    Ps = _rand_two_phase(rng, mats, 2)
+    Ps[17:20, 0] = Ps[2:5, 0]   # the outer state is at velocity equilibrium, so the frozen
+    Qs = ora.prim2cons(2, mats, Ps)  # end cells and their neighbours stay alike
+    nx = 160
+    Q0 = _piecewise(nx, Qs[:, [0, 1, 0]], np.array([0.4, 0.6]))
+    eos = _two_phase_eos(mats)
+    out = {}
+    for tau in (None, math.inf, 5e-3):
+        with Solver(eos, nx, flux="hll", cfl=0.6) as s:
+            if tau is not None:
+                s.set_relaxation(tau)
+            s.set_state(Q0)
+            s.run(steps=25, use_graph=False)
+            out[tau] = s.get_state()
+    assert np.array_equal(out[None], out[math.inf])
+    Q = out[5e-3]
+    assert not np.array_equal(Q, out[None])
+    for rows in ([1], [16], [2, 17], [3, 18], [4, 19], [5, 20]):
+        t0 = Q0[rows].sum()
+        t1 = Q[rows].sum()
+        assert abs(t1 - t0) <= 1e-12 * np.abs(Q0[rows]).sum(), rows
