@@ -55,12 +55,11 @@ __device__ __forceinline__ int batch_tile(const BatchArgs& a) {
   return __builtin_amdgcn_readfirstlane(a.tile_begin + w);
 }
 
+// Max of a positive CFL speed into a problem's slot.  Fire-and-forget: a read
+// of the slot first would keep every wave alive for one more memory round trip,
+// and a problem's slot sees only a few dozen waves per step.
 __device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double m) {
-  if (m > 0.0) {
-    unsigned long long bits = (unsigned long long)__double_as_longlong(m);
-    unsigned long long cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (bits > cur) atomicMax(slot, bits);
-  }
+  if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
 }
 
 template <int NPH, int FLUX, int FAST, int BLK>
