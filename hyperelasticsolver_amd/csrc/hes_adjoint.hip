@@ -484,14 +484,14 @@ void k_adj_step(AdjStepArgs a) {
       const double c = speed_and_grad(o + NQ, gO);
       const double u1 = o[20];
       const double lam = fabs(u1) + c;
-      const double lam_r = __shfl_down(lam, 1), lam_l = __shfl_up(lam, 1);
+      const double lam_r = lane_down(lam), lam_l = lane_up(lam);
       const double sR = fmax(lam, lam_r), sL = fmax(lam_l, lam);
       double sdR = 0.0, sdL = 0.0, fdot = 0.0;
 #pragma unroll
       for (int r = 0; r < NQ; ++r) {
         const double w = interior ? qb[r] : 0.0;
-        const double wr = __shfl_down(w, 1), wl = __shfl_up(w, 1);
-        const double qr = __shfl_down(q[r], 1), ql = __shfl_up(q[r], 1);
+        const double wr = lane_down(w), wl = lane_up(w);
+        const double qr = lane_down(q[r]), ql = lane_up(q[r]);
         const double gR = wr - w, gL = w - wl;
         const double gs = gL + gR;
         ob[r] = hdt * gs;

@@ -38,6 +38,30 @@ __device__ __forceinline__ void st_stream(double v, double* p) {
 #endif
 }
 
+// Neighbour values within a wave: lane_down(v) is lane + 1's v (lane 63 keeps
+// its own), lane_up(v) is lane - 1's v (lane 0 keeps its own), as
+// __shfl_down / __shfl_up by 1.  By default they are the wave-wide DPP shifts
+// (wave_shl:1 / wave_shr:1, plain VALU moves); HES_DPP_SHIFT=0 selects the LDS
+// crossbar (ds_bpermute) instead.
+#ifndef HES_DPP_SHIFT
+#define HES_DPP_SHIFT 1
+#endif
+#if HES_DPP_SHIFT
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v) {
+  const long long b = __double_as_longlong(v);
+  int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+  lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_down(double v) { return dpp_shift<0x130>(v); }  // wave_shl:1
+__device__ __forceinline__ double lane_up(double v) { return dpp_shift<0x138>(v); }    // wave_shr:1
+#else
+__device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1); }
+__device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1); }
+#endif
+
 // Block order across the 8 XCDs: the hardware deals consecutive blocks to
 // XCDs round-robin; remap so each XCD walks a contiguous stretch of cells and
 // the cache lines two neighbouring tiles share stay in one L2.
@@ -133,14 +157,14 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     // is formed from the same two rounded halves by both of its cells, so the
     // two cells see bit-identical face fluxes.
     double lam_c = fmax(fabs(smin), fabs(smax));  // maximum(abs.(eigvals))
-    double hr = 0.5 * fmax(lam_c, __shfl_down(lam_c, 1));
-    double hl = 0.5 * fmax(__shfl_up(lam_c, 1), lam_c);
+    double hr = 0.5 * fmax(lam_c, lane_down(lam_c));
+    double hl = 0.5 * fmax(lane_up(lam_c), lam_c);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
       double fa = 0.5 * f[r] + hr * q[r];
       double fb = 0.5 * f[r] - hl * q[r];
-      double Fr = fa + __shfl_down(fb, 1);
-      double Fl = __shfl_up(fa, 1) + fb;
+      double Fr = fa + lane_down(fb);
+      double Fl = lane_up(fa) + fb;
       upd[r] = q[r] - dtdx * (Fr - Fl);  // main.jl:59 with zero path integrals
     }
   } else {
@@ -149,10 +173,10 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
   double qn[NC], fn[NC];
 #pragma unroll
   for (int r = 0; r < NC; ++r) {
-    qn[r] = __shfl_down(q[r], 1);
-    fn[r] = __shfl_down(f[r], 1);
+    qn[r] = lane_down(q[r]);
+    fn[r] = lane_down(f[r]);
   }
-  double smin_n = __shfl_down(smin, 1), smax_n = __shfl_down(smax, 1);
+  double smin_n = lane_down(smin), smax_n = lane_down(smax);
 
   // face i+1/2 (this lane) and face i-1/2 (from lane-1)
   if (FLUX == FLUX_RUSANOV) {
@@ -162,9 +186,9 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     face_rusanov<NPH, FAST>(a.mat, q, qn, f, fn, fmax(lam_c, lam_n), cons, dmp);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
-      double cl = __shfl_up(cons[r], 1);
+      double cl = lane_up(cons[r]);
       double nfr = NPH == 2 ? dmp[r] : 0.0;
-      double nfl = NPH == 2 ? __shfl_up(dmp[r], 1) : 0.0;
+      double nfl = NPH == 2 ? lane_up(dmp[r]) : 0.0;
       upd[r] = q[r] - dtdx * ((cons[r] - cl) + (nfr + nfl));  // main.jl:59
     }
   } else {
@@ -172,7 +196,7 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     face_hll<NPH, FAST>(a.mat, q, qn, f, fn, smin, smax_n, dm, dp);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
-      double dpl = __shfl_up(dp[r], 1);
+      double dpl = lane_up(dp[r]);
       upd[r] = q[r] - dtdx * (dm[r] + dpl);  // conservative part is zero (NumFluxes.jl:82)
     }
   }
