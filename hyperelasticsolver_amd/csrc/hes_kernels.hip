@@ -51,6 +51,7 @@ struct StepArgs {
   long long n, pitch;
   double cfl, dx, t_end;
   double relax;               // two phases: 1 / tau of the velocity relaxation (0: none)
+  const double* dpath;        // two phases, Rusanov: precomputed half path integrals
   int step;                   // step index modulo 6 (ring phase)
   int hist_idx;
   MatDev mat[2];
@@ -112,7 +113,8 @@ __device__ __forceinline__ void step_epilogue(const StepArgs& a, double lam_new,
 // One chunk per wave.
 template <int NPH, int FLUX, int FAST>
 __global__ __launch_bounds__(BLOCK)
-__attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : 1))) void k_step(StepArgs a) {
+__attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU
+                                    : (FLUX == FLUX_RUSANOV ? 2 : 1)))) void k_step(StepArgs a) {
   constexpr int NC = NComp<NPH>::value;
   const int lane = threadIdx.x & 63;
   const long long wave = (xcd_block() * BLOCK + threadIdx.x) >> 6;
@@ -120,12 +122,25 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : 1))) void 
   double tcur;
   if (!step_begin(a, tcur)) return;
   const ChunkCtx ctx = {a.mat, a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, 0, a.n, a.cfl, a.dx,
-                        a.relax};
+                        a.relax, a.dpath};
   double q[NC], cc[NPH];
   load_chunk<NPH>(ctx, i, q, cc);
   const double lmax = step_lmax(a);
-  double lam = chunk_step<NPH, FLUX, FAST>(ctx, i, q, cc, lmax);
+  constexpr bool PB = NPH == 2 && FLUX == FLUX_RUSANOV;  // path integrals from k_path2
+  double lam = chunk_step<NPH, FLUX, FAST, PB>(ctx, i, q, cc, lmax);
   step_epilogue(a, lam, tcur, lmax);
+}
+
+// Two phases, Rusanov: the half path integrals of the faces with a volume-
+// fraction jump, one face per thread, before the step that uses them.  Waves
+// with no such face leave after reading the two volume fractions.
+template <int FAST>
+__global__ __launch_bounds__(BLOCK) void k_path2(StepArgs a) {
+  const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  double tcur;
+  if (!(a.tring[a.step & 1] < a.t_end)) return;  // the step is a no-op too
+  (void)tcur;
+  face_path2<FAST>(a.mat, a.Qin, (double*)a.dpath, i, a.n, a.pitch);
 }
 
 // Sound speeds of every cell of a state and the max CFL speed (main.jl:204-212).
@@ -417,6 +432,7 @@ struct hes_solver {
   MatDev mat[2];
   double cfl, dx;
   double relax;         // 1 / tau of the two-phase velocity relaxation (0: none)
+  double* dpath;        // two phases: half path integrals of the faces (ncomp x pitch)
   double* Q[2];
   double* C[2];
   unsigned long long* slots;
@@ -437,6 +453,11 @@ template <int NPH, int FLUX>
 void launch_step_t(const StepArgs& a, long long nchunks, bool fast, hipStream_t s) {
   constexpr int WPB = BLOCK / 64;
   long long nblocks = (nchunks + WPB - 1) / WPB;
+  if (NPH == 2 && FLUX == FLUX_RUSANOV) {
+    const unsigned nbp = (unsigned)((a.n + BLOCK - 1) / BLOCK);
+    if (fast) hipLaunchKernelGGL(k_path2<1>, dim3(nbp), dim3(BLOCK), 0, s, a);
+    else hipLaunchKernelGGL(k_path2<0>, dim3(nbp), dim3(BLOCK), 0, s, a);
+  }
   if (fast)
     hipLaunchKernelGGL((k_step<NPH, FLUX, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
   else
@@ -474,6 +495,7 @@ StepArgs make_args(hes_solver* h, long long s, double t_end, double* hist, int h
   a.dx = h->dx;
   a.t_end = t_end;
   a.relax = h->relax;
+  a.dpath = h->dpath;
   a.step = (int)(s % 6);
   a.hist_idx = hist_idx;
   a.mat[0] = h->mat[0];
@@ -563,6 +585,7 @@ int hes_solver_create(hes_solver** out, int nphase, int flux, long long ncells,
     e = hipMalloc(&h->Q[b], qbytes);
     if (e == hipSuccess) e = hipMalloc(&h->C[b], cbytes);
   }
+  if (e == hipSuccess && nphase == 2) e = hipMalloc(&h->dpath, qbytes);
   if (e == hipSuccess) e = hipMalloc(&h->slots, 4 * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMalloc(&h->tring, 2 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->status, 4 * sizeof(int));
@@ -595,6 +618,7 @@ int hes_solver_destroy(hes_solver* h) {
     if (h->C[b]) (void)hipFree(h->C[b]);
   }
   if (h->slots) (void)hipFree(h->slots);
+  if (h->dpath) (void)hipFree(h->dpath);
   if (h->tring) (void)hipFree(h->tring);
   if (h->status) (void)hipFree(h->status);
   if (h->dt_hist) (void)hipFree(h->dt_hist);

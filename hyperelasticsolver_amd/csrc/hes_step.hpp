@@ -93,6 +93,9 @@ struct ChunkCtx {
   long long pitch, lo, hi;
   double cfl, dx;
   double relax = 0.0;  // two phases: 1 / tau of the interphase velocity relaxation (0: none)
+  // two phases, Rusanov: half path integrals of the faces with a volume-fraction
+  // jump, precomputed by k_path2 (face i+1/2 at [r * pitch + i]); null: inline
+  const double* __restrict__ dpath = nullptr;
 };
 
 // Interphase velocity relaxation of a two-phase cell over a time dt (an
@@ -132,7 +135,7 @@ __device__ __forceinline__ void relax_velocity2(const MatDev* mat, double* q, do
   q[20] += 0.5 * m[1] * (kn[1] - ke[1]) + 0.5 * dk;
 }
 
-template <int NPH, int FLUX, int FAST>
+template <int NPH, int FLUX, int FAST, bool PATHBUF = false>
 __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, const double* q,
                                              const double* cc, double lmax,
                                              double* lam_cur = nullptr) {
@@ -166,6 +169,30 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
       double Fr = fa + lane_down(fb);
       double Fl = lane_up(fa) + fb;
       upd[r] = q[r] - dtdx * (Fr - Fl);  // main.jl:59 with zero path integrals
+    }
+  } else if (FLUX == FLUX_RUSANOV && NPH == 2 && PATHBUF) {
+    // Two phases with the half path integrals precomputed (k_path2): each
+    // component's face flux is formed (as face_rusanov forms it) and consumed
+    // in turn, so the neighbour state and the face arrays are never all live
+    // at once; the path terms are added only in waves that have a face with a
+    // volume-fraction jump.
+    const double lam_c = fmax(fabs(smin), fabs(smax));  // maximum(abs.(eigvals))
+    const double lam_n = fmax(fabs(lane_down(smin)), fabs(lane_down(smax)));
+    const double hl = 0.5 * fmax(lam_c, lam_n);
+    const bool jump = !((lane_down(q[0]) - q[0]) == 0.0 && (lane_down(q[15]) - q[15]) == 0.0) &&
+                      i >= a.lo && i < a.hi - 1;
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+      const double qn = lane_down(q[r]), fn = lane_down(f[r]);
+      const double cons = 0.5 * (f[r] + fn) - hl * (qn - q[r]);
+      upd[r] = q[r] - dtdx * (cons - lane_up(cons));  // main.jl:59, conservative part
+    }
+    if (__any(jump)) {
+#pragma unroll
+      for (int r = 0; r < NC; ++r) {
+        const double dmp = jump ? a.dpath[r * a.pitch + i] : 0.0;
+        upd[r] -= dtdx * (dmp + lane_up(dmp));  // D- of face i+1/2 + D+ of face i-1/2
+      }
     }
   } else {
 
@@ -224,6 +251,31 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
   int anybad = (owned && bad) || bad2;
   if (__any(anybad) && lane == 0) atomicOr(a.status, 1);
   return lam_new;
+}
+
+// Half path integral 1/2 int B dpsi of face i+1/2 of a two-phase state (the
+// Rusanov D- = D+ of NumFluxes.jl:35-59), for the faces with a volume-fraction
+// jump; other faces are left untouched (the step reads only jump faces).
+template <int FAST>
+__device__ __forceinline__ void face_path2(const MatDev* mat, const double* __restrict__ Q,
+                                           double* __restrict__ D, long long i, long long n,
+                                           long long pitch) {
+  const bool valid = i >= 0 && i < n - 1;
+  const long long ic = valid ? i : 0;
+  const double a0 = Q[ic], a0n = Q[ic + 1], a1 = Q[15 * pitch + ic], a1n = Q[15 * pitch + ic + 1];
+  const bool jump = valid && !((a0n - a0) == 0.0 && (a1n - a1) == 0.0);
+  if (!__any(jump)) return;
+  if (jump) {
+    double q[30], qn[30], d[30];
+#pragma unroll
+    for (int r = 0; r < 30; ++r) {
+      q[r] = Q[r * pitch + i];
+      qn[r] = Q[r * pitch + i + 1];
+    }
+    bint2<true, FAST>(mat, q, qn, d);
+#pragma unroll
+    for (int r = 0; r < 30; ++r) D[r * pitch + i] = 0.5 * d[r];
+  }
 }
 
 template <int NPH>
