@@ -219,23 +219,51 @@ __device__ __forceinline__ void face_rusanov(const MatDev* mat, const double* ql
   }
 }
 
-// HLL face (NumFluxes.jl:70-132): D- and D+ (the conservative part is zero).
-// smin_l = min eig of the left cell, smax_r = max eig of the right cell.
+// HLL wave-speed bounds and the coefficients of D- and D+ (NumFluxes.jl:88-92,
+// 111-131): s_l = min(0, min eig(Q_m), min eig(Q_l)), s_r = max(0, max eig(Q_m),
+// max eig(Q_r)); D- = cl J + clr dQ, D+ = cr J - clr dQ.
+struct HllCoef {
+  double s_l, s_r, isd, cl, cr, clr;
+};
+
+__device__ __forceinline__ HllCoef hll_coef(double mn_m, double mx_m, double smin_l,
+                                            double smax_r) {
+  HllCoef c;
+  c.s_l = fmin(0.0, fmin(mn_m, smin_l));
+  c.s_r = fmax(0.0, fmax(mx_m, smax_r));
+  const double sd = c.s_r - c.s_l;
+  c.isd = rcp(sd);
+  c.cl = -c.s_l * c.isd;
+  c.cr = c.s_r * c.isd;
+  c.clr = c.s_l * c.s_r * c.isd;
+  return c;
+}
+
+// Eigenvalue range of the mid state Q_m = (Q_l + Q_r) / 2 of a face.
 template <int NPH, int FAST = -1>
-__device__ __forceinline__ void face_hll(const MatDev* mat, const double* ql, const double* qr,
-                                         const double* fl, const double* fr, double smin_l,
-                                         double smax_r, double* dm, double* dp) {
+__device__ __forceinline__ void mid_speed_range(const MatDev* mat, const double* ql,
+                                                const double* qr, double& mn_m, double& mx_m) {
   constexpr int NC = NComp<NPH>::value;
   double qm[NC];
 #pragma unroll
   for (int r = 0; r < NC; ++r) qm[r] = 0.5 * (ql[r] + qr[r]);
-  double mn_m, mx_m;
   state_speed_range<NPH, FAST>(mat, qm, mn_m, mx_m);
-  double s_l = fmin(0.0, fmin(mn_m, smin_l));
-  double s_r = fmax(0.0, fmax(mx_m, smax_r));
-  double sd = s_r - s_l;
-  double isd = rcp(sd);
-  double cl = -s_l * isd, cr = s_r * isd, clr = s_l * s_r * isd;
+}
+
+// HLL face (NumFluxes.jl:70-132): D- and D+ (the conservative part is zero).
+// smin_l = min eig of the left cell, smax_r = max eig of the right cell.  When
+// coef is given it receives the coefficients used.
+template <int NPH, int FAST = -1>
+__device__ __forceinline__ void face_hll(const MatDev* mat, const double* ql, const double* qr,
+                                         const double* fl, const double* fr, double smin_l,
+                                         double smax_r, double* dm, double* dp,
+                                         HllCoef* coef = nullptr) {
+  constexpr int NC = NComp<NPH>::value;
+  double mn_m, mx_m;
+  mid_speed_range<NPH, FAST>(mat, ql, qr, mn_m, mx_m);
+  const HllCoef hc = hll_coef(mn_m, mx_m, smin_l, smax_r);
+  if (coef) *coef = hc;
+  const double s_l = hc.s_l, s_r = hc.s_r, isd = hc.isd, cl = hc.cl, cr = hc.cr, clr = hc.clr;
   if (NPH == 1) {
 #pragma unroll
     for (int r = 0; r < NC; ++r) {

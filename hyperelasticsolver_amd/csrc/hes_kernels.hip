@@ -51,7 +51,7 @@ struct StepArgs {
   long long n, pitch;
   double cfl, dx, t_end;
   double relax;               // two phases: 1 / tau of the velocity relaxation (0: none)
-  const double* dpath;        // two phases, Rusanov: precomputed half path integrals
+  const double* dpath;        // two phases: path terms of the jump faces (k_path2, k_path2_hll)
   int step;                   // step index modulo 6 (ring phase)
   int hist_idx;
   MatDev mat[2];
@@ -113,8 +113,7 @@ __device__ __forceinline__ void step_epilogue(const StepArgs& a, double lam_new,
 // One chunk per wave.
 template <int NPH, int FLUX, int FAST>
 __global__ __launch_bounds__(BLOCK)
-__attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU
-                                    : (FLUX == FLUX_RUSANOV ? 2 : 1)))) void k_step(StepArgs a) {
+__attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : 2))) void k_step(StepArgs a) {
   constexpr int NC = NComp<NPH>::value;
   const int lane = threadIdx.x & 63;
   const long long wave = (xcd_block() * BLOCK + threadIdx.x) >> 6;
@@ -126,7 +125,7 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU
   double q[NC], cc[NPH];
   load_chunk<NPH>(ctx, i, q, cc);
   const double lmax = step_lmax(a);
-  constexpr bool PB = NPH == 2 && FLUX == FLUX_RUSANOV;  // path integrals from k_path2
+  constexpr bool PB = NPH == 2;  // path integrals from k_path2 / k_path2_hll
   double lam = chunk_step<NPH, FLUX, FAST, PB>(ctx, i, q, cc, lmax);
   step_epilogue(a, lam, tcur, lmax);
 }
@@ -141,6 +140,13 @@ __global__ __launch_bounds__(BLOCK) void k_path2(StepArgs a) {
   if (!(a.tring[a.step & 1] < a.t_end)) return;  // the step is a no-op too
   (void)tcur;
   face_path2<FAST>(a.mat, a.Qin, (double*)a.dpath, i, a.n, a.pitch);
+}
+
+template <int FAST>
+__global__ __launch_bounds__(BLOCK) void k_path2_hll(StepArgs a) {
+  const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (!(a.tring[a.step & 1] < a.t_end)) return;  // the step is a no-op too
+  face_path2_hll<FAST>(a.mat, a.Qin, a.Cin, (double*)a.dpath, i, a.n, a.pitch);
 }
 
 // Sound speeds of every cell of a state and the max CFL speed (main.jl:204-212).
@@ -453,10 +459,15 @@ template <int NPH, int FLUX>
 void launch_step_t(const StepArgs& a, long long nchunks, bool fast, hipStream_t s) {
   constexpr int WPB = BLOCK / 64;
   long long nblocks = (nchunks + WPB - 1) / WPB;
-  if (NPH == 2 && FLUX == FLUX_RUSANOV) {
+  if (NPH == 2) {
     const unsigned nbp = (unsigned)((a.n + BLOCK - 1) / BLOCK);
-    if (fast) hipLaunchKernelGGL(k_path2<1>, dim3(nbp), dim3(BLOCK), 0, s, a);
-    else hipLaunchKernelGGL(k_path2<0>, dim3(nbp), dim3(BLOCK), 0, s, a);
+    if (FLUX == FLUX_RUSANOV) {
+      if (fast) hipLaunchKernelGGL(k_path2<1>, dim3(nbp), dim3(BLOCK), 0, s, a);
+      else hipLaunchKernelGGL(k_path2<0>, dim3(nbp), dim3(BLOCK), 0, s, a);
+    } else {
+      if (fast) hipLaunchKernelGGL(k_path2_hll<1>, dim3(nbp), dim3(BLOCK), 0, s, a);
+      else hipLaunchKernelGGL(k_path2_hll<0>, dim3(nbp), dim3(BLOCK), 0, s, a);
+    }
   }
   if (fast)
     hipLaunchKernelGGL((k_step<NPH, FLUX, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
@@ -585,7 +596,7 @@ int hes_solver_create(hes_solver** out, int nphase, int flux, long long ncells,
     e = hipMalloc(&h->Q[b], qbytes);
     if (e == hipSuccess) e = hipMalloc(&h->C[b], cbytes);
   }
-  if (e == hipSuccess && nphase == 2) e = hipMalloc(&h->dpath, qbytes);
+  if (e == hipSuccess && nphase == 2) e = hipMalloc(&h->dpath, 2 * qbytes);  // D-, D+ rows
   if (e == hipSuccess) e = hipMalloc(&h->slots, 4 * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMalloc(&h->tring, 2 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->status, 4 * sizeof(int));

@@ -144,6 +144,17 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
   const long long pitch = a.pitch;
   const bool inr = (i >= a.lo) && (i < a.hi);
 
+  // two phases, HLL, path terms precomputed: the mid-state eigenvalue range of
+  // face i+1/2 first, while only the state is live (its value does not depend
+  // on the order)
+  double mn_m = 0.0, mx_m = 0.0;
+  if (FLUX == FLUX_HLL && NPH == 2 && PATHBUF) {
+    double qm[NC];
+#pragma unroll
+    for (int r = 0; r < NC; ++r) qm[r] = 0.5 * (q[r] + lane_down(q[r]));
+    state_speed_range<NPH, FAST>(a.mat, qm, mn_m, mx_m);
+  }
+
   double f[NC], smin, smax;
   int bad;
   cell_flux<NPH, FAST>(a.mat, q, cc, f, smin, smax, bad);
@@ -192,6 +203,30 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
       for (int r = 0; r < NC; ++r) {
         const double dmp = jump ? a.dpath[r * a.pitch + i] : 0.0;
         upd[r] -= dtdx * (dmp + lane_up(dmp));  // D- of face i+1/2 + D+ of face i-1/2
+      }
+    }
+  } else if (FLUX == FLUX_HLL && NPH == 2 && PATHBUF) {
+    // Two phases, HLL: D-, D+ without path integrals, streamed per component
+    // (exact for faces without a volume-fraction jump, where all three path
+    // integrals of NumFluxes.jl:111-131 vanish); faces with a jump add the
+    // difference to the full D-, D+ precomputed by k_path2_hll.
+    const HllCoef hc = hll_coef(mn_m, mx_m, smin, lane_down(smax));
+    const bool jump = !((lane_down(q[0]) - q[0]) == 0.0 && (lane_down(q[15]) - q[15]) == 0.0) &&
+                      i >= a.lo && i < a.hi - 1;
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+      const double J = lane_down(f[r]) - f[r];
+      const double dq = lane_down(q[r]) - q[r];
+      const double dm = hc.cl * J + hc.clr * dq;
+      const double dp = hc.cr * J - hc.clr * dq;
+      upd[r] = q[r] - dtdx * (dm + lane_up(dp));  // conservative part is zero (NumFluxes.jl:82)
+    }
+    if (__any(jump)) {
+#pragma unroll
+      for (int r = 0; r < NC; ++r) {
+        const double ddm = jump ? a.dpath[r * a.pitch + i] : 0.0;
+        const double ddp = jump ? a.dpath[(NC + r) * a.pitch + i] : 0.0;
+        upd[r] -= dtdx * (ddm + lane_up(ddp));
       }
     }
   } else {
@@ -275,6 +310,48 @@ __device__ __forceinline__ void face_path2(const MatDev* mat, const double* __re
     bint2<true, FAST>(mat, q, qn, d);
 #pragma unroll
     for (int r = 0; r < 30; ++r) D[r * pitch + i] = 0.5 * d[r];
+  }
+}
+
+// Two phases, HLL: for the faces with a volume-fraction jump, the full D-, D+
+// (NumFluxes.jl:70-132) minus the path-free part cl J + clr dQ, cr J - clr dQ
+// (rows 0..29: D-, rows 30..59: D+ of face i+1/2).  C holds the stored sound
+// speeds (2 rows), as the step uses them.
+template <int FAST>
+__device__ __forceinline__ void face_path2_hll(const MatDev* mat, const double* __restrict__ Q,
+                                               const double* __restrict__ C,
+                                               double* __restrict__ D, long long i, long long n,
+                                               long long pitch) {
+  const bool valid = i >= 0 && i < n - 1;
+  const long long ic = valid ? i : 0;
+  const double a0 = Q[ic], a0n = Q[ic + 1], a1 = Q[15 * pitch + ic], a1n = Q[15 * pitch + ic + 1];
+  const bool jump = valid && !((a0n - a0) == 0.0 && (a1n - a1) == 0.0);
+  if (!__any(jump)) return;
+  if (jump) {
+    double q[30], qn[30], f[30], fn[30], dm[30], dp[30], cc[2], ccn[2];
+#pragma unroll
+    for (int r = 0; r < 30; ++r) {
+      q[r] = Q[r * pitch + i];
+      qn[r] = Q[r * pitch + i + 1];
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      cc[p] = C[p * pitch + i];
+      ccn[p] = C[p * pitch + i + 1];
+    }
+    double smin, smax, smin_n, smax_n;
+    int bad;
+    cell_flux<2, FAST>(mat, q, cc, f, smin, smax, bad);
+    cell_flux<2, FAST>(mat, qn, ccn, fn, smin_n, smax_n, bad);
+    HllCoef hc;
+    face_hll<2, FAST>(mat, q, qn, f, fn, smin, smax_n, dm, dp, &hc);
+#pragma unroll
+    for (int r = 0; r < 30; ++r) {
+      const double J = fn[r] - f[r];
+      const double dq = qn[r] - q[r];
+      D[r * pitch + i] = dm[r] - (hc.cl * J + hc.clr * dq);
+      D[(30 + r) * pitch + i] = dp[r] - (hc.cr * J - hc.clr * dq);
+    }
   }
 }
 
