@@ -147,8 +147,9 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
   // two phases, HLL, path terms precomputed: the mid-state eigenvalue range of
   // face i+1/2 first, while only the state is live (its value does not depend
   // on the order)
+  constexpr bool HLL_STREAM = FLUX == FLUX_HLL && (NPH == 1 || PATHBUF);
   double mn_m = 0.0, mx_m = 0.0;
-  if (FLUX == FLUX_HLL && NPH == 2 && PATHBUF) {
+  if (HLL_STREAM) {
     double qm[NC];
 #pragma unroll
     for (int r = 0; r < NC; ++r) qm[r] = 0.5 * (q[r] + lane_down(q[r]));
@@ -205,13 +206,16 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
         upd[r] -= dtdx * (dmp + lane_up(dmp));  // D- of face i+1/2 + D+ of face i-1/2
       }
     }
-  } else if (FLUX == FLUX_HLL && NPH == 2 && PATHBUF) {
-    // Two phases, HLL: D-, D+ without path integrals, streamed per component
-    // (exact for faces without a volume-fraction jump, where all three path
-    // integrals of NumFluxes.jl:111-131 vanish); faces with a jump add the
-    // difference to the full D-, D+ precomputed by k_path2_hll.
+  } else if (HLL_STREAM) {
+    // HLL: D-, D+ without path integrals, streamed per component (for one
+    // phase that is all of NumFluxes.jl:70-132; for two phases it is exact for
+    // faces without a volume-fraction jump, where all three path integrals of
+    // NumFluxes.jl:111-131 vanish, and faces with a jump add the difference to
+    // the full D-, D+ precomputed by k_path2_hll).
     const HllCoef hc = hll_coef(mn_m, mx_m, smin, lane_down(smax));
-    const bool jump = !((lane_down(q[0]) - q[0]) == 0.0 && (lane_down(q[15]) - q[15]) == 0.0) &&
+    const bool jump = NPH == 2 &&
+                      !((lane_down(q[0]) - q[0]) == 0.0 &&
+                        (lane_down(q[NPH == 2 ? 15 : 0]) - q[NPH == 2 ? 15 : 0]) == 0.0) &&
                       i >= a.lo && i < a.hi - 1;
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
@@ -221,7 +225,7 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
       const double dp = hc.cr * J - hc.clr * dq;
       upd[r] = q[r] - dtdx * (dm + lane_up(dp));  // conservative part is zero (NumFluxes.jl:82)
     }
-    if (__any(jump)) {
+    if (NPH == 2 && __any(jump)) {
 #pragma unroll
       for (int r = 0; r < NC; ++r) {
         const double ddm = jump ? a.dpath[r * a.pitch + i] : 0.0;
