@@ -136,9 +136,7 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : 2))) void 
 template <int FAST>
 __global__ __launch_bounds__(BLOCK) void k_path2(StepArgs a) {
   const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-  double tcur;
   if (!(a.tring[a.step & 1] < a.t_end)) return;  // the step is a no-op too
-  (void)tcur;
   face_path2<FAST>(a.mat, a.Qin, (double*)a.dpath, i, a.n, a.pitch);
 }
 
