@@ -20,6 +20,7 @@ HES_ERR_DOMAIN = -3
 
 FLUX_RUSANOV = 0
 FLUX_HLL = 1
+FLUX_LXF = 2
 
 
 class HesError(RuntimeError):

@@ -11,7 +11,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import FLUX_HLL, FLUX_RUSANOV, check, columns, lib, ptr
+from ._lib import check, columns, lib, ptr
 from .solver import FLUXES, materials_array, ncomp_of, nphase_of
 
 MODES = {"eager": 0, "graph": 1, "loop": 2}

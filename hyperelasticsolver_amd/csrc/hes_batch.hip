@@ -46,6 +46,7 @@ struct BatchArgs {
   int tile_begin, tile_end;
   int step;  // ring phase (step index modulo 6)
   int xcd;   // deal consecutive blocks to one XCD
+  int lxf;   // Rusanov kernels: Lax-Friedrichs face speed dx / dt (main.jl:50)
 };
 
 template <int BLK>
@@ -76,7 +77,8 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? 3 : 1))) void k_batch_step(BatchAr
   // the problem's materials are read in place (wave-uniform loads); a local
   // copy whose address is passed on would live in scratch memory
   const MatDev* m = a.mats + NPH * p;
-  const ChunkCtx ctx = {m, a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, lo, hi, a.cfl[p], a.dx[p]};
+  const ChunkCtx ctx = {m,  a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, lo, hi, a.cfl[p], a.dx[p],
+                        0.0, nullptr, a.lxf};
   const long long i = first + lane - 1;
   double q[NC], cc[NPH];
   load_chunk<NPH>(ctx, i, q, cc);  // in flight while the problem's time is read
@@ -227,6 +229,7 @@ BatchArgs batch_args(hes_batch* h, long long s, int tb, int te) {
   a.tile_end = te;
   a.step = (int)(s % 6);
   a.xcd = h->xcd;
+  a.lxf = h->flux == HES_FLUX_LXF;
   return a;
 }
 
@@ -242,15 +245,16 @@ void launch_batch_t(const BatchArgs& a, int block, hipStream_t st) {
 }
 
 void launch_batch(hes_batch* h, const BatchArgs& a) {
+  // Lax-Friedrichs runs the Rusanov kernels with the fixed face speed (a.lxf)
   if (h->nph == 1) {
-    if (h->flux == HES_FLUX_RUSANOV)
+    if (h->flux != HES_FLUX_HLL)
       h->fast ? launch_batch_t<1, FLUX_RUSANOV, 1>(a, h->block, h->stream)
               : launch_batch_t<1, FLUX_RUSANOV, 0>(a, h->block, h->stream);
     else
       h->fast ? launch_batch_t<1, FLUX_HLL, 1>(a, h->block, h->stream)
               : launch_batch_t<1, FLUX_HLL, 0>(a, h->block, h->stream);
   } else {
-    if (h->flux == HES_FLUX_RUSANOV)
+    if (h->flux != HES_FLUX_HLL)
       h->fast ? launch_batch_t<2, FLUX_RUSANOV, 1>(a, h->block, h->stream)
               : launch_batch_t<2, FLUX_RUSANOV, 0>(a, h->block, h->stream);
     else
@@ -313,7 +317,8 @@ int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const
   if (!out) return fail(HES_ERR_ARG, "null handle pointer");
   *out = nullptr;
   if (nphase != 1 && nphase != 2) return fail(HES_ERR_ARG, "nphase must be 1 or 2");
-  if (flux != HES_FLUX_RUSANOV && flux != HES_FLUX_HLL) return fail(HES_ERR_ARG, "unknown flux");
+  if (flux != HES_FLUX_RUSANOV && flux != HES_FLUX_HLL && flux != HES_FLUX_LXF)
+    return fail(HES_ERR_ARG, "unknown flux");
   if (nproblems < 1 || !ncells || !materials) return fail(HES_ERR_ARG, "null argument");
   hes_batch* h = new hes_batch();
   h->nph = nphase;

@@ -52,6 +52,7 @@ struct StepArgs {
   double cfl, dx, t_end;
   double relax;               // two phases: 1 / tau of the velocity relaxation (0: none)
   const double* dpath;        // two phases: path terms of the jump faces (k_path2, k_path2_hll)
+  int lxf;                    // Rusanov kernels: Lax-Friedrichs face speed dx / dt (main.jl:50)
   int step;                   // step index modulo 6 (ring phase)
   int hist_idx;
   MatDev mat[2];
@@ -121,7 +122,7 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : 2))) void 
   double tcur;
   if (!step_begin(a, tcur)) return;
   const ChunkCtx ctx = {a.mat, a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, 0, a.n, a.cfl, a.dx,
-                        a.relax, a.dpath};
+                        a.relax, a.dpath, a.lxf};
   double q[NC], cc[NPH];
   load_chunk<NPH>(ctx, i, q, cc);
   const double lmax = step_lmax(a);
@@ -478,9 +479,11 @@ bool all_fast(const hes_solver* h);
 void launch_step(hes_solver* h, const StepArgs& a) {
   long long nwaves = (h->n + TILE - 1) / TILE;
   bool fast = all_fast(h);
-  if (h->nph == 1 && h->flux == FLUX_RUSANOV) launch_step_t<1, FLUX_RUSANOV>(a, nwaves, fast, h->stream);
+  // Lax-Friedrichs runs the Rusanov kernels with the fixed face speed (a.lxf)
+  const bool hll = h->flux == HES_FLUX_HLL;
+  if (h->nph == 1 && !hll) launch_step_t<1, FLUX_RUSANOV>(a, nwaves, fast, h->stream);
   else if (h->nph == 1) launch_step_t<1, FLUX_HLL>(a, nwaves, fast, h->stream);
-  else if (h->flux == FLUX_RUSANOV) launch_step_t<2, FLUX_RUSANOV>(a, nwaves, fast, h->stream);
+  else if (!hll) launch_step_t<2, FLUX_RUSANOV>(a, nwaves, fast, h->stream);
   else launch_step_t<2, FLUX_HLL>(a, nwaves, fast, h->stream);
 }
 
@@ -505,6 +508,7 @@ StepArgs make_args(hes_solver* h, long long s, double t_end, double* hist, int h
   a.t_end = t_end;
   a.relax = h->relax;
   a.dpath = h->dpath;
+  a.lxf = h->flux == HES_FLUX_LXF;
   a.step = (int)(s % 6);
   a.hist_idx = hist_idx;
   a.mat[0] = h->mat[0];
@@ -569,7 +573,8 @@ int hes_solver_create(hes_solver** out, int nphase, int flux, long long ncells,
   if (!out) return fail(HES_ERR_ARG, "null handle pointer");
   *out = nullptr;
   if (nphase != 1 && nphase != 2) return fail(HES_ERR_ARG, "nphase must be 1 or 2");
-  if (flux != HES_FLUX_RUSANOV && flux != HES_FLUX_HLL) return fail(HES_ERR_ARG, "unknown flux");
+  if (flux != HES_FLUX_RUSANOV && flux != HES_FLUX_HLL && flux != HES_FLUX_LXF)
+    return fail(HES_ERR_ARG, "unknown flux");
   if (ncells < 3) return fail(HES_ERR_ARG, "need at least 3 cells");
   if (!materials) return fail(HES_ERR_ARG, "null materials");
   for (int p = 0; p < nphase; ++p)
@@ -1045,6 +1050,7 @@ int hes_invariants(const double* G, double* inv, long long n) {
 int hes_face_flux(int nph, int flux, const double* mats, const double* QL, const double* QR,
                   const double* aux, double* cons, double* dm, double* dp, long long n) {
   if (nph != 1 && nph != 2) return fail(HES_ERR_ARG, "nphase must be 1 or 2");
+  if (flux == HES_FLUX_LXF) flux = HES_FLUX_RUSANOV;  // one face: the same lxf, lambda given in aux
   if (flux != HES_FLUX_RUSANOV && flux != HES_FLUX_HLL) return fail(HES_ERR_ARG, "unknown flux");
   if (!mats || !QL || !QR || !aux || !cons || !dm || !dp || n < 0)
     return fail(HES_ERR_ARG, "null argument");

@@ -96,6 +96,10 @@ struct ChunkCtx {
   // two phases, Rusanov: half path integrals of the faces with a volume-fraction
   // jump, precomputed by k_path2 (face i+1/2 at [r * pitch + i]); null: inline
   const double* __restrict__ dpath = nullptr;
+  // Rusanov kernels: nonzero selects the Lax-Friedrichs face speed
+  // lambda = 1 / dtdx = dx / dt on every face (main.jl:30-41, 50) in place of
+  // max(lambda_i, lambda_i+1); the path integrals do not depend on it
+  int lxf = 0;
 };
 
 // Interphase velocity relaxation of a two-phase cell over a time dt (an
@@ -171,9 +175,10 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     // a_i + b_i+1 with a = F/2 + lam/2 Q and b = F/2 - lam/2 Q.  Each face sum
     // is formed from the same two rounded halves by both of its cells, so the
     // two cells see bit-identical face fluxes.
-    double lam_c = fmax(fabs(smin), fabs(smax));  // maximum(abs.(eigvals))
-    double hr = 0.5 * fmax(lam_c, lane_down(lam_c));
-    double hl = 0.5 * fmax(lane_up(lam_c), lam_c);
+    const double lam_c = fmax(fabs(smin), fabs(smax));  // maximum(abs.(eigvals))
+    const double hg = 0.5 * (1.0 / dtdx);                  // main.jl:50
+    const double hr = a.lxf ? hg : 0.5 * fmax(lam_c, lane_down(lam_c));
+    const double hl = a.lxf ? hg : 0.5 * fmax(lane_up(lam_c), lam_c);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
       double fa = 0.5 * f[r] + hr * q[r];
@@ -190,7 +195,7 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     // volume-fraction jump.
     const double lam_c = fmax(fabs(smin), fabs(smax));  // maximum(abs.(eigvals))
     const double lam_n = fmax(fabs(lane_down(smin)), fabs(lane_down(smax)));
-    const double hl = 0.5 * fmax(lam_c, lam_n);
+    const double hl = a.lxf ? 0.5 * (1.0 / dtdx) : 0.5 * fmax(lam_c, lam_n);  // main.jl:47-50
     const bool jump = !((lane_down(q[0]) - q[0]) == 0.0 && (lane_down(q[15]) - q[15]) == 0.0) &&
                       i >= a.lo && i < a.hi - 1;
 #pragma unroll
@@ -249,7 +254,8 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     double lam_c = fmax(fabs(smin), fabs(smax));       // maximum(abs.(eigvals))
     double lam_n = fmax(fabs(smin_n), fabs(smax_n));
     double cons[NC], dmp[NC];
-    face_rusanov<NPH, FAST>(a.mat, q, qn, f, fn, fmax(lam_c, lam_n), cons, dmp);
+    face_rusanov<NPH, FAST>(a.mat, q, qn, f, fn, a.lxf ? 1.0 / dtdx : fmax(lam_c, lam_n), cons,
+                            dmp);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
       double cl = lane_up(cons[r]);

@@ -11,9 +11,12 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import FLUX_HLL, FLUX_RUSANOV, check, columns, lib, ptr
+from ._lib import FLUX_HLL, FLUX_LXF, FLUX_RUSANOV, check, columns, lib, ptr
 
-FLUXES = {"rusanov": FLUX_RUSANOV, "lxf": FLUX_RUSANOV, "hll": FLUX_HLL}
+# flux kinds of the time step: the face speed of NumFluxes.lxf as main.jl:46-50
+# chooses it (Rusanov: max over the face's two cells; Lax-Friedrichs: dx / dt),
+# or NumFluxes.hll (main.jl:56-57)
+FLUXES = {"rusanov": FLUX_RUSANOV, "hll": FLUX_HLL, "lxf": FLUX_LXF, "lax_friedrichs": FLUX_LXF}
 
 
 def materials_array(eos):
@@ -40,8 +43,9 @@ class Solver:
 
     eos   : Barton2009 (one phase, 14-component states) or a tuple of two
             (two phases, 30-component states, HyperelasticityMPh layout)
-    flux  : "rusanov" (lxf with the local max speed, main.jl:46-53) or "hll"
-            (main.jl:56-57)
+    flux  : "rusanov" (lxf with the local max speed, main.jl:46-48), "hll"
+            (main.jl:56-57) or "lxf" / "lax_friedrichs" (lxf with
+            lambda = dx / dt on every face, main.jl:30-41, 50)
     """
 
     def __init__(self, eos, nx, flux="hll", cfl=0.6, X=1.0, dx=None):

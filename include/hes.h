@@ -36,6 +36,8 @@ extern "C" {
 
 #define HES_FLUX_RUSANOV 0 /* lxf with lambda = max(lambda_l, lambda_r), main.jl:46-53 */
 #define HES_FLUX_HLL 1     /* hll, main.jl:56-57 */
+#define HES_FLUX_LXF 2     /* Lax-Friedrichs: lxf with lambda = dx / dt on every face,
+                              main.jl:30-41 and the variant of main.jl:50 */
 
 typedef struct hes_solver hes_solver;
 

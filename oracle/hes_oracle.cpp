@@ -681,7 +681,7 @@ inline void hll(const Model& m, const double* Ql, const double* Qr, const double
   }
 }
 
-enum { FLUX_RUSANOV = 0, FLUX_HLL = 1 };
+enum { FLUX_RUSANOV = 0, FLUX_HLL = 1, FLUX_LXF = 2 };
 
 // main.jl:202-241 time loop; Q is ncomp x nx column-major (component fastest).
 // Stops after max_steps steps or once t >= t_end (checked before each step).
@@ -726,9 +726,11 @@ inline int run(const Model& m, int flux, int nx, double* Q, double cfl, double d
       if (flux == FLUX_HLL) {  // main.jl:56-59
         hll(m, Ql, Qc, eig.data() + (size_t)ne * (i - 1), eig.data() + (size_t)ne * i, Fl, dml, NFl);
         hll(m, Qc, Qr, eig.data() + (size_t)ne * i, eig.data() + (size_t)ne * (i + 1), Fr, NFr, dpr);
-      } else {  // Rusanov, main.jl:46-53 (commented variant)
-        double lam_l = std::max(lam[i - 1], lam[i]);
-        double lam_r = std::max(lam[i], lam[i + 1]);
+      } else {  // Rusanov, main.jl:46-53 (commented variants)
+        // Rusanov: lambda_l, lambda_r = max over the face's two cells (main.jl:47-48);
+        // Lax-Friedrichs: lambda_l, lambda_r = 1 / dtdx, 1 / dtdx (main.jl:50)
+        double lam_l = flux == FLUX_LXF ? 1 / dtdx : std::max(lam[i - 1], lam[i]);
+        double lam_r = flux == FLUX_LXF ? 1 / dtdx : std::max(lam[i], lam[i + 1]);
         lxf(m, Ql, Qc, lam_l, Fl, dml, NFl);
         lxf(m, Qc, Qr, lam_r, Fr, NFr, dpr);
       }

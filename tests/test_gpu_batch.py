@@ -89,7 +89,7 @@ def test_batch_matches_oracle(ora):
     print("[batch] vs oracle", flush=True)
     ncells = [64, 150, 333]
     steps = 12
-    for flux, fk in (("rusanov", 0), ("hll", 1)):
+    for flux, fk in (("rusanov", 0), ("hll", 1), ("lxf", 2)):
         b, eos, sts, ifs = _build(ora, ncells, flux=flux, seed=2)
         Q0 = b.get_state()
         b.run(steps, mode="eager")
@@ -102,17 +102,18 @@ def test_batch_matches_oracle(ora):
             assert np.all(err <= TOL), (flux, p, err.max())
 
 
-def test_batch_two_phase_matches_oracle(ora):
+@pytest.mark.parametrize("flux,fk", [("hll", 1), ("lxf", 2)])
+def test_batch_two_phase_matches_oracle(ora, flux, fk):
     from hyperelasticsolver_amd.solver import materials_array
-    print("[batch] two-phase vs oracle", flush=True)
+    print(f"[batch] two-phase vs oracle, flux={flux}", flush=True)
     ncells = [40, 70]
-    b, eos, sts, ifs = _build(ora, ncells, nph=2, flux="hll", seed=3)
+    b, eos, sts, ifs = _build(ora, ncells, nph=2, flux=flux, seed=3)
     Q0 = b.get_state()
     b.run(5, mode="graph")
     Qb = b.get_state()
     b.close()
     for p, n in enumerate(ncells):
-        Qo, _, _, _ = ora.run(2, materials_array(eos[p]), 1, Q0[:, b.offsets[p]:b.offsets[p + 1]],
+        Qo, _, _, _ = ora.run(2, materials_array(eos[p]), fk, Q0[:, b.offsets[p]:b.offsets[p + 1]],
                               0.6, 1.0 / n, steps=5)
         err = rel_err_per_component(Qb[:, b.offsets[p]:b.offsets[p + 1]], Qo)
         assert np.all(err <= TOL), (p, err.max())

@@ -69,3 +69,30 @@ def test_driver_restart_resumes_at_the_snapshot_time(tmp_path):
     err = rel_err_per_component(Qr, Qf)
     rows = [r for r in range(30) if r not in (1, 16)]
     assert np.all(err[rows] <= 1e-10), err
+
+
+def test_initial_condition_tanh(ora):
+    """main.jl:110-123: the left/right primitives by side, and the volume
+    fraction a1 = 0.1 (tanh(4 (x - 1/2) / eps) + 1) + 0.4, a2 = 1 - a1."""
+    from hyperelasticsolver_amd import driver
+    from hyperelasticsolver_amd import HyperelasticityMPh as H
+    print("[driver] initial_condition_tanh, test case 6, 50 cells", flush=True)
+    eos = _eos()
+    mats = np.concatenate([eos[0].params(), eos[1].params()])
+    Ql, Qr = H.initial_states(eos, 6)
+    nx, eps = 50, 0.1
+    Q = driver.initial_condition_tanh(eos, Ql, Qr, nx, eps)
+    x = (np.arange(1, nx + 1) - 0.5) / nx
+    a1 = 0.1 * (np.tanh(4 * (x - 0.5) / eps) + 1) + 0.4
+    P = H.cons2prim_mph(eos, Q)
+    np.testing.assert_allclose(P[0], a1, rtol=1e-13)
+    np.testing.assert_allclose(P[15], 1 - a1, rtol=1e-13)
+    Pl, Pr = H.cons2prim_mph(eos, Ql), H.cons2prim_mph(eos, Qr)
+    side = np.where((x < 0.5)[None, :], Pl[:, None], Pr[:, None])
+    rows = [r for r in range(30) if r not in (0, 15)]
+    np.testing.assert_allclose(P[rows], side[rows], rtol=1e-12, atol=1e-15)
+    # the conserved states are the oracle's prim2cons of the same primitives
+    Pexp = side.copy()
+    Pexp[0], Pexp[15] = a1, 1 - a1
+    Qo = ora.prim2cons(2, mats, np.asfortranarray(Pexp))
+    assert np.all(rel_err_per_component(Q, Qo) <= 1e-12)

@@ -15,6 +15,7 @@ from oracle_bind import rel_err_per_component
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
+FK = {"rusanov": 0, "hll": 1, "lxf": 2}   # the oracle's flux kinds
 
 MAT_CU = np.array([8.93, 4.6, 3.9e-4, 300.0, 2.1, 1.0, 3.0, 2.0])     # Barton2009() defaults
 MAT_AL = np.array([2.7, 4.6, 3.9e-4, 300.0, 3.1, 1.0, 3.0, 2.0])      # second phase of the 2-phase case
@@ -82,11 +83,13 @@ def _gpu_run(nph, mats, flux, Q0, steps, use_graph=False, t_end=math.inf):
     (1, "hll", 300, 60, 4),
     (2, "rusanov", 64, 12, 3),
     (2, "hll", 64, 12, 3),
+    (1, "lxf", 300, 60, 4),
+    (2, "lxf", 64, 12, 3),
 ])
 def test_step_parity(ora, nph, flux, nx, steps, nstates):
     print(f"[parity] nph={nph} flux={flux} nx={nx} steps={steps}", flush=True)
     mats, Q0, _, _ = _problem(ora, nph, nx, nstates, seed=100 + nph)
-    fk = 0 if flux == "rusanov" else 1
+    fk = FK[flux]
     Qo, to, so, dto = ora.run(nph, mats, fk, Q0, 0.6, 1.0 / nx, steps=steps)
     Qg, r = _gpu_run(nph, mats, flux, Q0, steps)
     assert r["steps"] == so == steps
@@ -126,7 +129,7 @@ def test_uniform_state_is_bit_invariant(ora, nph):
     print(f"[uniform] nph={nph}", flush=True)
     mats, _, Qs, _ = _problem(ora, nph, 10, 1, seed=3)
     Q0 = np.asfortranarray(np.repeat(Qs[:, :1], 130, axis=1))
-    for flux in ("rusanov", "hll"):
+    for flux in ("rusanov", "hll", "lxf"):
         Qg, r = _gpu_run(nph, mats, flux, Q0, 7)
         assert r["steps"] == 7
         assert np.array_equal(Qg, Q0)
@@ -139,7 +142,7 @@ def test_conservation_telescopes(ora):
     mats, _, Qs, _ = _problem(ora, 1, 10, 3, seed=5)
     nx = 400
     Q0 = _piecewise(nx, Qs[:, [0, 1, 0]], np.array([0.4, 0.6]))
-    for flux in ("rusanov", "hll"):
+    for flux in ("rusanov", "hll", "lxf"):
         Qg, r = _gpu_run(1, mats, flux, Q0, 50)
         tot0 = Q0.sum(axis=1)
         tot1 = Qg.sum(axis=1)
@@ -147,14 +150,14 @@ def test_conservation_telescopes(ora):
         assert np.all(np.abs(tot1 - tot0) <= 1e-13 * scale + 1e-300), (tot1 - tot0) / scale
 
 
-@pytest.mark.parametrize("flux", ["rusanov", "hll"])
+@pytest.mark.parametrize("flux", ["rusanov", "hll", "lxf"])
 def test_parity_many_blocks(ora, flux):
     """2^16 cells: ~1000 blocks spread over all XCDs, so the CFL max-reduction
     across blocks and the XCD block order are exercised."""
     nx, steps = 1 << 16, 6
     print(f"[parity-large] flux={flux} nx={nx} steps={steps}", flush=True)
     mats, Q0, _, _ = _problem(ora, 1, nx, 33, seed=404)
-    fk = 0 if flux == "rusanov" else 1
+    fk = FK[flux]
     Qo, to, so, dto = ora.run(1, mats, fk, Q0, 0.6, 1.0 / nx, steps=steps)
     Qg, r = _gpu_run(1, mats, flux, Q0, steps)
     err = rel_err_per_component(Qg, Qo)

@@ -340,7 +340,7 @@ def _riemann(ora, nph, mats, nx, seed):
     return np.asfortranarray(np.stack([Ql if i < nx / 2 else Qr for i in range(nx)], axis=1))
 
 
-@pytest.mark.parametrize("nph,flux", [(1, 0), (1, 1), (2, 0), (2, 1)])
+@pytest.mark.parametrize("nph,flux", [(1, 0), (1, 1), (1, 2), (2, 0), (2, 1), (2, 2)])
 def test_uniform_state_stays_bit_identical(ora, nph, flux):
     mats = MAT if nph == 1 else MAT2
     rng = np.random.default_rng(16)
@@ -350,7 +350,7 @@ def test_uniform_state_stays_bit_identical(ora, nph, flux):
     assert s == 5 and np.array_equal(Q1, Q0)
 
 
-@pytest.mark.parametrize("flux", [0, 1])
+@pytest.mark.parametrize("flux", [0, 1, 2])
 def test_identical_phases_at_half_are_half_the_single_phase_run(ora, flux):
     """HyperelasticityMPh test cases 4/5 replicate the single-phase cases with two
     identical phases at a = 1/2: every non-conservative term is multiplied by a
@@ -368,6 +368,47 @@ def test_identical_phases_at_half_are_half_the_single_phase_run(ora, flux):
     for p in range(2):
         assert np.all(R2[15 * p] == 0.5)
         assert np.array_equal(R2[15 * p + 1:15 * p + 15], 0.5 * R1)
+
+
+def test_lax_friedrichs_is_the_textbook_scheme(ora):
+    """With lambda = dx / dt on every face (main.jl:30-41, 50) and no path
+    term, the update is the Lax-Friedrichs scheme in its textbook form
+    Q_i <- (Q_i-1 + Q_i+1) / 2 - dt / (2 dx) (F_i+1 - F_i-1)."""
+    nx = 32
+    Q0 = _riemann(ora, 1, MAT, nx, seed=21)
+    Q0[:, 9:13] = ora.prim2cons(1, MAT, _rand_prims(np.random.default_rng(22), 1, MAT))[:, None]
+    Q1, t, s, dt = ora.run(1, MAT, 2, Q0, 0.6, 1.0 / nx, steps=1)
+    dtdx = dt[0] * nx
+    F = ora.flux(1, MAT, Q0)
+    ref = 0.5 * (Q0[:, :-2] + Q0[:, 2:]) - 0.5 * dtdx * (F[:, 2:] - F[:, :-2])
+    scale = np.abs(Q0).max(axis=1) + dtdx * np.abs(F).max(axis=1)
+    assert np.all(np.abs(Q1[:, 1:-1] - ref) <= 1e-14 * scale[:, None])
+    assert np.array_equal(Q1[:, [0, -1]], Q0[:, [0, -1]])  # frozen ends (main.jl:219-220)
+    # and it differs from the Rusanov step (a face speed below dx / dt)
+    Qr, _, _, dtr = ora.run(1, MAT, 0, Q0, 0.6, 1.0 / nx, steps=1)
+    assert dtr[0] == dt[0] and not np.array_equal(Qr, Q1)
+
+
+def test_lax_friedrichs_and_rusanov_differ_by_the_face_speed(ora):
+    """Two phases: the path integrals do not depend on lambda, so one step of
+    each differs exactly by the viscosity term with the two face speeds,
+    dt/dx [(L - lam_r) (Q_i+1 - Q_i) - (L - lam_l) (Q_i - Q_i-1)] / 2 with
+    L = dx / dt and lam the max |eigenvalue| over the face's two cells."""
+    nx = 24
+    Q0 = _riemann(ora, 2, MAT2, nx, seed=23)
+    Q0[:, 5:8] = ora.prim2cons(2, MAT2, _rand_prims(np.random.default_rng(24), 2, MAT2))[:, None]
+    QL, _, _, dt = ora.run(2, MAT2, 2, Q0, 0.6, 1.0 / nx, steps=1)
+    QR, _, _, dtr = ora.run(2, MAT2, 0, Q0, 0.6, 1.0 / nx, steps=1)
+    assert dt[0] == dtr[0]
+    dtdx = dt[0] * nx
+    lam = np.abs(ora.get_eigvals(2, MAT2, Q0)).max(axis=0)
+    lf = np.maximum(lam[:-1], lam[1:])              # face i+1/2, i = 0..nx-2
+    dQ = Q0[:, 1:] - Q0[:, :-1]
+    L = 1.0 / dtdx
+    vis = (L - lf) * dQ                              # per face
+    ref = 0.5 * dtdx * (vis[:, 1:] - vis[:, :-1])    # interior cells 1..nx-2
+    scale = np.abs(Q0).max(axis=1)
+    assert np.all(np.abs((QL - QR)[:, 1:-1] - ref) <= 1e-13 * scale[:, None])
 
 
 def test_conservation_and_t_end(ora):
