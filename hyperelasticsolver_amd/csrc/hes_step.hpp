@@ -57,9 +57,29 @@ __device__ __forceinline__ double dpp_shift(double v) {
 }
 __device__ __forceinline__ double lane_down(double v) { return dpp_shift<0x130>(v); }  // wave_shl:1
 __device__ __forceinline__ double lane_up(double v) { return dpp_shift<0x138>(v); }    // wave_shr:1
+// The same shifts writing zero into the lane that has no source (lane 63 for
+// lane_down_z, lane 0 for lane_up_z) or whose source lane is inactive, so
+// they are only used where every lane of the wave is active: never under a
+// lane-dependent condition (a && or ?: whose other operand varies by lane).  With no old value to keep, the shifted
+// copy goes straight into a new register: one DPP move per half, where
+// lane_down / lane_up first copy the value (the original stays live).  For
+// values whose boundary-lane result only reaches the halo lanes' own face and
+// update, which are never stored.
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift_z(double v) {
+  const long long b = __double_as_longlong(v);
+  int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+  lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, true);
+  hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_down_z(double v) { return dpp_shift_z<0x130>(v); }
+__device__ __forceinline__ double lane_up_z(double v) { return dpp_shift_z<0x138>(v); }
 #else
 __device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1); }
 __device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1); }
+__device__ __forceinline__ double lane_down_z(double v) { return __shfl_down(v, 1); }
+__device__ __forceinline__ double lane_up_z(double v) { return __shfl_up(v, 1); }
 #endif
 
 // Block order across the 8 XCDs: the hardware deals consecutive blocks to
@@ -75,6 +95,7 @@ __device__ __forceinline__ long long xcd_block() {
 #ifndef HES_STEP_WAVES_PER_EU
 #define HES_STEP_WAVES_PER_EU 3
 #endif
+
 
 // One 64-lane chunk: lane l holds cell i = first + l - 1 (lanes 0 and 63 are
 // halo cells); q and cc are its state and sound speeds.  Updates the owned
@@ -177,14 +198,15 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     // two cells see bit-identical face fluxes.
     const double lam_c = fmax(fabs(smin), fabs(smax));  // maximum(abs.(eigvals))
     const double hg = 0.5 * (1.0 / dtdx);                  // main.jl:50
-    const double hr = a.lxf ? hg : 0.5 * fmax(lam_c, lane_down(lam_c));
-    const double hl = a.lxf ? hg : 0.5 * fmax(lane_up(lam_c), lam_c);
+    const double lam_r = lane_down_z(lam_c), lam_l = lane_up_z(lam_c);
+    const double hr = a.lxf ? hg : 0.5 * fmax(lam_c, lam_r);
+    const double hl = a.lxf ? hg : 0.5 * fmax(lam_l, lam_c);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
       double fa = 0.5 * f[r] + hr * q[r];
       double fb = 0.5 * f[r] - hl * q[r];
-      double Fr = fa + lane_down(fb);
-      double Fl = lane_up(fa) + fb;
+      double Fr = fa + lane_down_z(fb);
+      double Fl = lane_up_z(fa) + fb;
       upd[r] = q[r] - dtdx * (Fr - Fl);  // main.jl:59 with zero path integrals
     }
   } else if (FLUX == FLUX_RUSANOV && NPH == 2 && PATHBUF) {
@@ -196,8 +218,11 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     const double lam_c = fmax(fabs(smin), fabs(smax));  // maximum(abs.(eigvals))
     const double lam_n = fmax(fabs(lane_down(smin)), fabs(lane_down(smax)));
     const double hl = a.lxf ? 0.5 * (1.0 / dtdx) : 0.5 * fmax(lam_c, lam_n);  // main.jl:47-50
-    const bool jump = !((lane_down(q[0]) - q[0]) == 0.0 && (lane_down(q[15]) - q[15]) == 0.0) &&
-                      i >= a.lo && i < a.hi - 1;
+    // the shifts run with every lane active (not under the lane-dependent
+    // conditions), so each lane sees its right neighbour's volume fractions
+    const double a0n = lane_down(q[0]), a1n = lane_down(q[15]);
+    const bool jump = !(((a0n - q[0]) == 0.0) & ((a1n - q[15]) == 0.0)) & (i >= a.lo) &
+                      (i < a.hi - 1);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
       const double qn = lane_down(q[r]), fn = lane_down(f[r]);
@@ -218,10 +243,12 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     // NumFluxes.jl:111-131 vanish, and faces with a jump add the difference to
     // the full D-, D+ precomputed by k_path2_hll).
     const HllCoef hc = hll_coef(mn_m, mx_m, smin, lane_down(smax));
-    const bool jump = NPH == 2 &&
-                      !((lane_down(q[0]) - q[0]) == 0.0 &&
-                        (lane_down(q[NPH == 2 ? 15 : 0]) - q[NPH == 2 ? 15 : 0]) == 0.0) &&
-                      i >= a.lo && i < a.hi - 1;
+    bool jump = false;
+    if (NPH == 2) {  // shifts with every lane active, as in the Rusanov branch
+      const double a0n = lane_down(q[0]), a1n = lane_down(q[NPH == 2 ? 15 : 0]);
+      jump = !(((a0n - q[0]) == 0.0) & ((a1n - q[NPH == 2 ? 15 : 0]) == 0.0)) & (i >= a.lo) &
+             (i < a.hi - 1);
+    }
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
       const double J = lane_down(f[r]) - f[r];
