@@ -23,7 +23,9 @@
 //
 // Storage: every `every`-th state is kept (with its sound speeds); a backward
 // segment recomputes the states in between with the forward kernel (bit-identical
-// to the first pass), then runs the reverse steps over them.  The lmax term of
+// to the first pass), then runs the reverse steps over them.  The recompute of
+// the next (earlier) segment runs on a second stream into the other of two
+// segment buffers while the reverse steps of the current one run.  The lmax term of
 // step s needs the global sum dtdxbar_s, so it is applied when step s-1 loads
 // Qbar_s: the one or two waves holding the argmax cell reduce the block
 // partials of step s and add lmaxbar_s * dlam/d(Q, b0, c0) of that cell
@@ -657,7 +659,7 @@ struct hes_adjoint {
   MatDev mat;
   int nst;
   double* ck = nullptr;    // nck slots of 15 x pitch (Q, then C)
-  double* seg = nullptr;   // every - 1 slots of 14 x pitch
+  double* seg[2] = {nullptr, nullptr};  // two segment buffers of every - 1 slots of 14 x pitch
   double* fq[2] = {nullptr, nullptr};
   double* fc[2] = {nullptr, nullptr};
   double* qb[2] = {nullptr, nullptr};
@@ -683,6 +685,11 @@ struct hes_adjoint {
   int nslots = 0;
   int* status = nullptr;
   hipStream_t stream = nullptr;
+  // the backward pass recomputes segment c - 1 on stream2 while the reverse
+  // steps of segment c run on stream (see record_backward)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_ready[2] = {nullptr, nullptr},
+             ev_done[2] = {nullptr, nullptr};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t gf = nullptr, gb = nullptr;
   bool has_init = false, has_target = false, fwd_done = false;
@@ -704,10 +711,11 @@ void fwd_slot(hes_adjoint* h, long long s, double** q, double** c) {
   }
 }
 
-// State slot of state s inside its backward segment (s0 = first state).
-double* seg_slot(hes_adjoint* h, long long s, long long s0) {
+// State slot of state s inside its backward segment (s0 = first state); the
+// segment's recomputed states live in segment buffer `buf`.
+double* seg_slot(hes_adjoint* h, long long s, long long s0, int buf) {
   if (s == s0) return ck_q(h, s0 / h->every);
-  return h->seg + (s - s0 - 1) * 14 * h->pitch;
+  return h->seg[buf] + (s - s0 - 1) * 14 * h->pitch;
 }
 
 AdjFwdArgs fwd_args(hes_adjoint* h, long long s, const double* qi, double* qo, const double* ci,
@@ -763,6 +771,13 @@ hipError_t record_forward(hes_adjoint* h) {
   return hipGetLastError();
 }
 
+// Returns a failed HIP call's status from a recording function.
+#define HIP_TRY_E(expr)            \
+  do {                             \
+    hipError_t e_ = (expr);        \
+    if (e_ != hipSuccess) return e_; \
+  } while (0)
+
 hipError_t record_backward(hes_adjoint* h) {
   hipStream_t st = h->stream;
   hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((2 * h->nb + 255) / 256)), dim3(256), 0, st,
@@ -773,22 +788,47 @@ hipError_t record_backward(hes_adjoint* h) {
                      h->zf + (long long)fin * h->nw, h->n, h->pitch, h->nw);
   hipLaunchKernelGGL(k_adj_izero, dim3((unsigned)((h->nw + 255) / 256)), dim3(256), 0, st,
                      h->zf + (long long)(1 - fin) * h->nw, (long long)h->nw);
-  for (long long c = h->nck - 1; c >= 0; --c) {
+  // Segment c's states are recomputed into segment buffer c & 1 on stream2
+  // (the k_adj_fwd steps and k_adj_lam, which read only the checkpoints and
+  // the first pass's CFL records), while the reverse steps of segment c + 1
+  // run on `st` from the other buffer.  Events order the two: the reverse of
+  // segment c waits for its buffer to be ready, and the recompute of segment
+  // c - 1 waits until the reverse of segment c + 1 (the last reader of the
+  // buffer it overwrites) is done.  Both streams are in the capture: stream2
+  // forks from `st` and joins it at the end.
+  hipStream_t st2 = h->stream2;
+  HIP_TRY_E(hipEventRecord(h->ev_fork, st));
+  HIP_TRY_E(hipStreamWaitEvent(st2, h->ev_fork, 0));
+  auto recompute = [&](long long c) -> hipError_t {
     const long long s0 = c * h->every;
     const long long end = std::min(s0 + h->every, h->nsteps);
-    // recompute states s0+1 .. end-1
+    const int buf = (int)(c & 1);
     for (long long s = s0; s + 1 < end; ++s) {
       const double* ci = s == s0 ? ck_c(h, c) : h->fc[(s - s0) & 1];
       double* co = h->fc[(s - s0 + 1) & 1];
-      hipLaunchKernelGGL(k_adj_fwd, dim3((unsigned)h->nb), dim3(ABLK), 0, st,
-                         fwd_args(h, s, seg_slot(h, s, s0), seg_slot(h, s + 1, s0), ci, co, false));
+      hipLaunchKernelGGL(k_adj_fwd, dim3((unsigned)h->nb), dim3(ABLK), 0, st2,
+                         fwd_args(h, s, seg_slot(h, s, s0, buf), seg_slot(h, s + 1, s0, buf), ci,
+                                  co, false));
     }
     const int cnt = (int)(end - s0);
-    hipLaunchKernelGGL(k_adj_lam, dim3((cnt + 63) / 64), dim3(64), 0, st, ck_q(h, c), h->seg,
-                       14 * h->pitch, h->pitch, h->istar, h->gin, s0, cnt, h->n, h->Jlam);
+    hipLaunchKernelGGL(k_adj_lam, dim3((cnt + 63) / 64), dim3(64), 0, st2, ck_q(h, c),
+                       h->seg[buf], 14 * h->pitch, h->pitch, h->istar, h->gin, s0, cnt, h->n,
+                       h->Jlam);
+    return hipEventRecord(h->ev_ready[buf], st2);
+  };
+  HIP_TRY_E(recompute(h->nck - 1));
+  for (long long c = h->nck - 1; c >= 0; --c) {
+    const long long s0 = c * h->every;
+    const long long end = std::min(s0 + h->every, h->nsteps);
+    const int buf = (int)(c & 1);
+    if (c >= 1) {
+      if (c + 1 < h->nck) HIP_TRY_E(hipStreamWaitEvent(st2, h->ev_done[(c + 1) & 1], 0));
+      HIP_TRY_E(recompute(c - 1));
+    }
+    HIP_TRY_E(hipStreamWaitEvent(st, h->ev_ready[buf], 0));
     for (long long s = end - 1; s >= s0; --s) {
       AdjStepArgs a;
-      a.Q = seg_slot(h, s, s0);
+      a.Q = seg_slot(h, s, s0, buf);
       a.Qb_in = h->qb[(s + 1) & 1];
       a.Qb_out = h->qb[s & 1];
       a.zin = h->zf + ((s + 1) & 1) * h->nw;
@@ -815,7 +855,10 @@ hipError_t record_backward(hes_adjoint* h) {
       a.gin[4] = h->mat8[3];
       hipLaunchKernelGGL(k_adj_step, dim3((unsigned)h->nb), dim3(ABLK), 0, st, a);
     }
+    HIP_TRY_E(hipEventRecord(h->ev_done[buf], st));
   }
+  HIP_TRY_E(hipEventRecord(h->ev_join, st2));
+  HIP_TRY_E(hipStreamWaitEvent(st, h->ev_join, 0));
   hipLaunchKernelGGL(k_adj_q0sum, dim3((unsigned)h->nbl), dim3(LBLK), 0, st, h->qb[0], h->bounds,
                      h->nst, h->part, h->n, h->pitch);
   hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(64), 0, st, h->part, h->nbl, h->thb, h->nb,
@@ -874,10 +917,16 @@ int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* materi
   h->nst = 0;
   const size_t P = (size_t)h->pitch;
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);
   if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  hipEvent_t* sync_events[] = {&h->ev_fork, &h->ev_join, &h->ev_ready[0], &h->ev_ready[1],
+                               &h->ev_done[0], &h->ev_done[1]};
+  for (hipEvent_t* ev : sync_events)
+    if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
   if (e == hipSuccess) e = dalloc(&h->ck, (size_t)h->nck * 15 * P);
-  if (e == hipSuccess && h->every > 1) e = dalloc(&h->seg, (size_t)(h->every - 1) * 14 * P);
+  for (int b = 0; b < 2 && e == hipSuccess && h->every > 1; ++b)
+    e = dalloc(&h->seg[b], (size_t)(h->every - 1) * 14 * P);
   for (int b = 0; b < 2 && e == hipSuccess; ++b) {
     e = dalloc(&h->fq[b], 14 * P);
     if (e == hipSuccess) e = dalloc(&h->fc[b], P);
@@ -906,7 +955,8 @@ int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* materi
     // every state slot whose end cells must be filled
     std::vector<double*> sl;
     for (long long c = 1; c < h->nck; ++c) sl.push_back(ck_q(h, c));
-    for (long long j = 0; j + 1 < h->every; ++j) sl.push_back(h->seg + j * 14 * P);
+    for (int b = 0; b < 2; ++b)
+      for (long long j = 0; j + 1 < h->every; ++j) sl.push_back(h->seg[b] + j * 14 * P);
     sl.push_back(h->fq[0]);
     sl.push_back(h->fq[1]);
     h->nslots = (int)sl.size();
@@ -927,16 +977,20 @@ int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* materi
 int hes_adjoint_destroy(hes_adjoint* h) {
   if (!h) return HES_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->stream2) (void)hipStreamSynchronize(h->stream2);
   if (h->gf) (void)hipGraphExecDestroy(h->gf);
   if (h->gb) (void)hipGraphExecDestroy(h->gb);
-  void* bufs[] = {h->ck, h->seg, h->fq[0], h->fq[1], h->fc[0], h->fc[1], h->qb[0], h->qb[1], h->zf,
+  void* bufs[] = {h->ck, h->seg[0], h->seg[1], h->fq[0], h->fq[1], h->fc[0], h->fc[1], h->qb[0], h->qb[1], h->zf,
                   h->lmax, h->blk_val, h->blk_idx, h->istar, h->Jlam, h->dtp, h->thb, h->lossp,
                   h->part, h->sums, h->gout, h->gin, h->prims, h->states, h->bounds, h->rho_t,
                   h->u_t, h->slots, h->status};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
-  if (h->ev0) (void)hipEventDestroy(h->ev0);
-  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  hipEvent_t evs[] = {h->ev0, h->ev1, h->ev_fork, h->ev_join, h->ev_ready[0], h->ev_ready[1],
+                      h->ev_done[0], h->ev_done[1]};
+  for (hipEvent_t ev : evs)
+    if (ev) (void)hipEventDestroy(ev);
+  if (h->stream2) (void)hipStreamDestroy(h->stream2);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return HES_OK;
@@ -1124,7 +1178,7 @@ int hes_adjoint_get_profiles(hes_adjoint* h, double* rho, double* u1) {
 int hes_adjoint_info(hes_adjoint* h, long long* device_bytes, long long* checkpoints) {
   if (!h) return fail(HES_ERR_ARG, "null handle");
   const long long P = h->pitch;
-  long long b = h->nck * 15 * P + (h->every - 1) * 14 * P + 2 * (14 + 1 + 14) * P;
+  long long b = h->nck * 15 * P + 2 * (h->every - 1) * 14 * P + 2 * (14 + 1 + 14) * P;
   b = b * 8 + (h->nsteps + 1) * 8 + h->nsteps * (long long)h->nb * 16 + h->nsteps * (8 + JL * 8);
   if (device_bytes) *device_bytes = b;
   if (checkpoints) *checkpoints = h->nck;
