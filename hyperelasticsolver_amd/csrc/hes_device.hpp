@@ -72,6 +72,35 @@ __device__ __forceinline__ double det3cm(const double* a) {
          a[6] * (a[1] * a[5] - a[4] * a[2]);
 }
 
+// Density of a phase block q = [a r, a r u(3), a r E, a r A(9)]: rho =
+// sqrt(det(a r A) / a^3 / rho0), as cons2prim forms it (the stored a r is not
+// read), den = a rho and iden = 1 / den; bad: non-positive det.
+template <bool ONE_FRAC>
+__device__ __forceinline__ void phase_density(const MatDev& m, double frac, const double* q,
+                                              double& rho, double& den, double& iden, int& bad) {
+  const double detP = det3cm(q + 5);
+  double rho2;
+  if (ONE_FRAC) {
+    rho2 = detP * m.irho0;
+  } else {
+    double ifr = rcp(frac);
+    rho2 = detP * (ifr * ifr * ifr) * m.irho0;
+  }
+  bad = !(detP > 0.0);
+  rho = sqrt(rho2);
+  den = ONE_FRAC ? rho : frac * rho;
+  iden = rcp(den);
+}
+
+// Normal velocity u1 of a phase block (the u[0] that thermo computes).
+template <bool ONE_FRAC>
+__device__ __forceinline__ double phase_u1(const MatDev& m, double frac, const double* q) {
+  double rho, den, iden;
+  int bad;
+  phase_density<ONE_FRAC>(m, frac, q, rho, den, iden, bad);
+  return q[1] * iden;
+}
+
 // Finger tensor, invariants and energy split of a phase.
 // q points at [a r, a r u(3), a r E, a r A(9)] (the phase block without a).
 // HyperelasticityMPh.jl:106-124 (cons2prim) + Strains.jl:26-52 + EquationsOfState.jl:139-156.
@@ -80,18 +109,8 @@ __device__ __forceinline__ double det3cm(const double* a) {
 template <bool ONE_FRAC, int FAST = -1>
 __device__ __forceinline__ void thermo(const MatDev& m, double frac, const double* q, Thermo& t) {
   const double* P = q + 5;
-  double detP = det3cm(P);
-  double rho2;
-  if (ONE_FRAC) {
-    rho2 = detP * m.irho0;
-  } else {
-    double ifr = rcp(frac);
-    rho2 = detP * (ifr * ifr * ifr) * m.irho0;
-  }
-  t.bad = !(detP > 0.0);
-  t.rho = sqrt(rho2);
-  t.den = ONE_FRAC ? t.rho : frac * t.rho;
-  double iden = rcp(t.den);
+  double iden;
+  phase_density<ONE_FRAC>(m, frac, q, t.rho, t.den, iden, t.bad);
   t.u[0] = q[1] * iden;
   t.u[1] = q[2] * iden;
   t.u[2] = q[3] * iden;

@@ -96,6 +96,13 @@ __device__ __forceinline__ long long xcd_block() {
 #define HES_STEP_WAVES_PER_EU 3
 #endif
 
+// Keeps a value's computation where it stands (emits no instruction).  The
+// updated state is consumed only in the branch of interior cells; left alone,
+// the compiler sinks the update arithmetic into that branch and every face
+// value and shifted neighbour value stays live across it instead of one
+// component at a time.
+__device__ __forceinline__ void pin(double& v) { asm volatile("" : "+v"(v)); }
+
 
 // One 64-lane chunk: lane l holds cell i = first + l - 1 (lanes 0 and 63 are
 // halo cells); q and cc are its state and sound speeds.  Updates the owned
@@ -169,12 +176,28 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
   const long long pitch = a.pitch;
   const bool inr = (i >= a.lo) && (i < a.hi);
 
-  // two phases, HLL, path terms precomputed: the mid-state eigenvalue range of
-  // face i+1/2 first, while only the state is live (its value does not depend
-  // on the order)
+  // HLL (one phase, or two with the path terms precomputed): the mid-state
+  // eigenvalue range of face i+1/2 first, while only the state is live (its
+  // value does not depend on the order)
   constexpr bool HLL_STREAM = FLUX == FLUX_HLL && (NPH == 1 || PATHBUF);
+  // two phases with the path terms precomputed: the face speeds come first
+  // (from u1 and the stored sound speeds), then each phase's flux is formed
+  // and consumed by the update in turn, so only one phase's flux is live
+  constexpr bool PHASED = NPH == 2 && PATHBUF;
   double mn_m = 0.0, mx_m = 0.0;
-  if (HLL_STREAM) {
+  if (HLL_STREAM && PHASED) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      double qm[15];
+#pragma unroll
+      for (int r = 0; r < 15; ++r) qm[r] = 0.5 * (q[15 * p + r] + lane_down(q[15 * p + r]));
+      Thermo t;
+      thermo<false, FAST>(a.mat[p], qm[0], qm + 1, t);
+      const double c = sound_speed(a.mat[p], t);
+      mn_m = p == 0 ? t.u[0] - c : fmin(mn_m, t.u[0] - c);
+      mx_m = p == 0 ? t.u[0] + c : fmax(mx_m, t.u[0] + c);
+    }
+  } else if (HLL_STREAM) {
     double qm[NC];
 #pragma unroll
     for (int r = 0; r < NC; ++r) qm[r] = 0.5 * (q[r] + lane_down(q[r]));
@@ -183,7 +206,18 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
 
   double f[NC], smin, smax;
   int bad;
-  cell_flux<NPH, FAST>(a.mat, q, cc, f, smin, smax, bad);
+  if (PHASED) {
+    // [min_p(u1 - c_p), max_p(u1 + c_p)] as cell_flux forms it
+    bad = 0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const double u1 = phase_u1<false>(a.mat[p], q[15 * p], q + 15 * p + 1);
+      smin = p == 0 ? u1 - cc[p] : fmin(smin, u1 - cc[p]);
+      smax = p == 0 ? u1 + cc[p] : fmax(smax, u1 + cc[p]);
+    }
+  } else {
+    cell_flux<NPH, FAST>(a.mat, q, cc, f, smin, smax, bad);
+  }
 
   // dt is needed only from here on: the CFL maximum is read after the state
   // loads are in flight (main.jl:212, dt/dx of main.jl:225)
@@ -224,16 +258,36 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     const bool jump = !(((a0n - q[0]) == 0.0) & ((a1n - q[15]) == 0.0)) & (i >= a.lo) &
                       (i < a.hi - 1);
 #pragma unroll
-    for (int r = 0; r < NC; ++r) {
-      const double qn = lane_down(q[r]), fn = lane_down(f[r]);
-      const double cons = 0.5 * (f[r] + fn) - hl * (qn - q[r]);
-      upd[r] = q[r] - dtdx * (cons - lane_up(cons));  // main.jl:59, conservative part
+    for (int p = 0; p < 2; ++p) {
+      Thermo t;
+      thermo<false, FAST>(a.mat[p], q[15 * p], q + 15 * p + 1, t);
+      bad |= t.bad;
+      double fp[15];
+      fp[0] = 0.0;
+      phase_flux(a.mat[p], q[15 * p], t, fp + 1);
+#pragma unroll
+      for (int k = 0; k < 15; ++k) {
+        const int r = 15 * p + k;
+        const double qn = lane_down(q[r]), fn = lane_down(fp[k]);
+        const double cons = 0.5 * (fp[k] + fn) - hl * (qn - q[r]);
+        upd[r] = q[r] - dtdx * (cons - lane_up(cons));  // main.jl:59, conservative part
+        pin(upd[r]);
+      }
     }
     if (__any(jump)) {
+      // every lane loads (from a cell index clamped into the problem) and
+      // then selects, so the 30 loads are issued together rather than one
+      // masked load at a time; every lane is active (the branch is
+      // wave-uniform) and lane 0's update is never stored
+      const long long ic = i < a.lo ? a.lo : (i >= a.hi ? a.hi - 1 : i);
+      double d[NC];
+#pragma unroll
+      for (int r = 0; r < NC; ++r) d[r] = a.dpath[r * a.pitch + ic];
 #pragma unroll
       for (int r = 0; r < NC; ++r) {
-        const double dmp = jump ? a.dpath[r * a.pitch + i] : 0.0;
-        upd[r] -= dtdx * (dmp + lane_up(dmp));  // D- of face i+1/2 + D+ of face i-1/2
+        const double dmp = jump ? d[r] : 0.0;
+        upd[r] -= dtdx * (dmp + lane_up_z(dmp));  // D- of face i+1/2 + D+ of face i-1/2
+        pin(upd[r]);
       }
     }
   } else if (HLL_STREAM) {
@@ -249,20 +303,48 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
       jump = !(((a0n - q[0]) == 0.0) & ((a1n - q[NPH == 2 ? 15 : 0]) == 0.0)) & (i >= a.lo) &
              (i < a.hi - 1);
     }
-#pragma unroll
-    for (int r = 0; r < NC; ++r) {
-      const double J = lane_down(f[r]) - f[r];
+    auto stream = [&](int r, double fr) {
+      const double J = lane_down(fr) - fr;
       const double dq = lane_down(q[r]) - q[r];
       const double dm = hc.cl * J + hc.clr * dq;
       const double dp = hc.cr * J - hc.clr * dq;
       upd[r] = q[r] - dtdx * (dm + lane_up(dp));  // conservative part is zero (NumFluxes.jl:82)
-    }
-    if (NPH == 2 && __any(jump)) {
+      if (PHASED) pin(upd[r]);
+    };
+    if (PHASED) {
 #pragma unroll
-      for (int r = 0; r < NC; ++r) {
-        const double ddm = jump ? a.dpath[r * a.pitch + i] : 0.0;
-        const double ddp = jump ? a.dpath[(NC + r) * a.pitch + i] : 0.0;
-        upd[r] -= dtdx * (ddm + lane_up(ddp));
+      for (int p = 0; p < 2; ++p) {
+        Thermo t;
+        thermo<false, FAST>(a.mat[p], q[15 * p], q + 15 * p + 1, t);
+        bad |= t.bad;
+        double fp[15];
+        fp[0] = 0.0;
+        phase_flux(a.mat[p], q[15 * p], t, fp + 1);
+#pragma unroll
+        for (int k = 0; k < 15; ++k) stream(15 * p + k, fp[k]);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < NC; ++r) stream(r, f[r]);
+    }
+    if (NPH == 2 && __any(jump)) {  // loads and shifts as in the Rusanov branch, a phase at a time
+      const long long ic = i < a.lo ? a.lo : (i >= a.hi ? a.hi - 1 : i);
+#pragma unroll
+      for (int p = 0; p < NPH; ++p) {
+        double dm[15], dp[15];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) {
+          dm[k] = a.dpath[(15 * p + k) * a.pitch + ic];
+          dp[k] = a.dpath[(NC + 15 * p + k) * a.pitch + ic];
+        }
+#pragma unroll
+        for (int k = 0; k < 15; ++k) {
+          const int r = 15 * p + k;
+          const double ddm = jump ? dm[k] : 0.0;
+          const double ddp = jump ? dp[k] : 0.0;
+          upd[r] -= dtdx * (ddm + lane_up_z(ddp));
+          pin(upd[r]);
+        }
       }
     }
   } else {
