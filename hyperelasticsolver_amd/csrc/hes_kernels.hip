@@ -55,6 +55,8 @@ struct StepArgs {
   double cfl, dx, t_end;
   double relax;               // two phases: 1 / tau of the velocity relaxation (0: none)
   const double* dpath;        // two phases: path terms of the jump faces (k_path2, k_path2_hll)
+  unsigned long long* jcnt;   // two phases: jump-face count of step s in jcnt[s & 1]
+  long long* jlist;           // two phases: the jump faces of the step (k_mark2)
   int lxf;                    // Rusanov kernels: Lax-Friedrichs face speed dx / dt (main.jl:50)
   int step;                   // step index modulo 6 (ring phase)
   int hist_idx;
@@ -134,21 +136,46 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : HES_STEP2_
   step_epilogue(a, lam, tcur, lmax);
 }
 
-// Two phases, Rusanov: the half path integrals of the faces with a volume-
-// fraction jump, one face per thread, before the step that uses them.  Waves
-// with no such face leave after reading the two volume fractions.
-template <int FAST>
-__global__ __launch_bounds__(BLOCK) void k_path2(StepArgs a) {
+// Two phases: the faces with a volume-fraction jump of the step's input state,
+// appended to a list (face index i of face i+1/2) whose length is counted in
+// jcnt[step & 1]; the other slot is cleared here for the next step.  One face
+// per thread, a ballot and one atomic per wave.
+__global__ __launch_bounds__(BLOCK) void k_mark2(StepArgs a) {
   const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.jcnt[(a.step + 1) & 1] = 0ull;
   if (!(a.tring[a.step & 1] < a.t_end)) return;  // the step is a no-op too
-  face_path2<FAST>(a.mat, a.Qin, (double*)a.dpath, i, a.n, a.pitch);
+  const bool jump = i < a.n - 1 && face_jump2(a.Qin, i, a.pitch);
+  const unsigned long long m = __ballot(jump);
+  if (m == 0ull) return;
+  unsigned long long base = 0ull;
+  if (lane == 0) base = atomicAdd(a.jcnt + (a.step & 1), (unsigned long long)__popcll(m));
+  base = __shfl(base, 0);
+  if (jump) a.jlist[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
+}
+
+// The path terms of the listed jump faces over a fixed grid: a group of 16
+// lanes per face, four faces per wave (the faces without a jump never reach
+// this kernel).
+template <int FAST>
+__global__ __launch_bounds__(64) void k_path2(StepArgs a) {
+  if (!(a.tring[a.step & 1] < a.t_end)) return;
+  const unsigned long long cnt = a.jcnt[a.step & 1];
+  const unsigned long long stride = (unsigned long long)gridDim.x * 4;
+  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 4; j < cnt;
+       j += stride)
+    face_path2<FAST>(a.mat, a.Qin, (double*)a.dpath, a.jlist[j], a.pitch, threadIdx.x & 15);
 }
 
 template <int FAST>
-__global__ __launch_bounds__(BLOCK) void k_path2_hll(StepArgs a) {
-  const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-  if (!(a.tring[a.step & 1] < a.t_end)) return;  // the step is a no-op too
-  face_path2_hll<FAST>(a.mat, a.Qin, a.Cin, (double*)a.dpath, i, a.n, a.pitch);
+__global__ __launch_bounds__(64) void k_path2_hll(StepArgs a) {
+  if (!(a.tring[a.step & 1] < a.t_end)) return;
+  const unsigned long long cnt = a.jcnt[a.step & 1];
+  const unsigned long long stride = (unsigned long long)gridDim.x * 4;
+  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 4; j < cnt;
+       j += stride)
+    face_path2_hll<FAST>(a.mat, a.Qin, a.Cin, (double*)a.dpath, a.jlist[j], a.pitch,
+                         threadIdx.x & 15);
 }
 
 // Sound speeds of every cell of a state and the max CFL speed (main.jl:204-212).
@@ -441,6 +468,8 @@ struct hes_solver {
   double cfl, dx;
   double relax;         // 1 / tau of the two-phase velocity relaxation (0: none)
   double* dpath;        // two phases: half path integrals of the faces (ncomp x pitch)
+  unsigned long long* jcnt;  // two phases: jump-face counts (2 slots, by step parity)
+  long long* jlist;          // two phases: jump-face list (n entries)
   double* Q[2];
   double* C[2];
   unsigned long long* slots;
@@ -462,13 +491,17 @@ void launch_step_t(const StepArgs& a, long long nchunks, bool fast, hipStream_t 
   constexpr int WPB = BLOCK / 64;
   long long nblocks = (nchunks + WPB - 1) / WPB;
   if (NPH == 2) {
+    // jump faces first (a light scan), then their path terms over a fixed
+    // grid of at most one wave per SIMD
     const unsigned nbp = (unsigned)((a.n + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL(k_mark2, dim3(nbp), dim3(BLOCK), 0, s, a);
+    const unsigned nbl = (unsigned)std::min<long long>((a.n + 254) / 256, 1024);  // 4 faces per block
     if (FLUX == FLUX_RUSANOV) {
-      if (fast) hipLaunchKernelGGL(k_path2<1>, dim3(nbp), dim3(BLOCK), 0, s, a);
-      else hipLaunchKernelGGL(k_path2<0>, dim3(nbp), dim3(BLOCK), 0, s, a);
+      if (fast) hipLaunchKernelGGL(k_path2<1>, dim3(nbl), dim3(64), 0, s, a);
+      else hipLaunchKernelGGL(k_path2<0>, dim3(nbl), dim3(64), 0, s, a);
     } else {
-      if (fast) hipLaunchKernelGGL(k_path2_hll<1>, dim3(nbp), dim3(BLOCK), 0, s, a);
-      else hipLaunchKernelGGL(k_path2_hll<0>, dim3(nbp), dim3(BLOCK), 0, s, a);
+      if (fast) hipLaunchKernelGGL(k_path2_hll<1>, dim3(nbl), dim3(64), 0, s, a);
+      else hipLaunchKernelGGL(k_path2_hll<0>, dim3(nbl), dim3(64), 0, s, a);
     }
   }
   if (fast)
@@ -511,6 +544,8 @@ StepArgs make_args(hes_solver* h, long long s, double t_end, double* hist, int h
   a.t_end = t_end;
   a.relax = h->relax;
   a.dpath = h->dpath;
+  a.jcnt = h->jcnt;
+  a.jlist = h->jlist;
   a.lxf = h->flux == HES_FLUX_LXF;
   a.step = (int)(s % 6);
   a.hist_idx = hist_idx;
@@ -523,6 +558,7 @@ StepArgs make_args(hes_solver* h, long long s, double t_end, double* hist, int h
 int init_speeds(hes_solver* h) {
   long long s = h->step;
   HIP_TRY(hipMemsetAsync(h->slots, 0, 3 * sizeof(unsigned long long), h->stream));
+  if (h->jcnt) HIP_TRY(hipMemsetAsync(h->jcnt, 0, 2 * sizeof(unsigned long long), h->stream));
   double tr[2] = {h->t, h->t};
   HIP_TRY(hipMemcpyAsync(h->tring, tr, sizeof(tr), hipMemcpyHostToDevice, h->stream));
   long long nb = (h->n + BLOCK - 1) / BLOCK;
@@ -603,6 +639,8 @@ int hes_solver_create(hes_solver** out, int nphase, int flux, long long ncells,
     if (e == hipSuccess) e = hipMalloc(&h->C[b], cbytes);
   }
   if (e == hipSuccess && nphase == 2) e = hipMalloc(&h->dpath, 2 * qbytes);  // D-, D+ rows
+  if (e == hipSuccess && nphase == 2) e = hipMalloc(&h->jcnt, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess && nphase == 2) e = hipMalloc(&h->jlist, sizeof(long long) * (size_t)ncells);
   if (e == hipSuccess) e = hipMalloc(&h->slots, 4 * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMalloc(&h->tring, 2 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->status, 4 * sizeof(int));
@@ -636,6 +674,8 @@ int hes_solver_destroy(hes_solver* h) {
   }
   if (h->slots) (void)hipFree(h->slots);
   if (h->dpath) (void)hipFree(h->dpath);
+  if (h->jcnt) (void)hipFree(h->jcnt);
+  if (h->jlist) (void)hipFree(h->jlist);
   if (h->tring) (void)hipFree(h->tring);
   if (h->status) (void)hipFree(h->status);
   if (h->dt_hist) (void)hipFree(h->dt_hist);

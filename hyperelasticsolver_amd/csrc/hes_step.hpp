@@ -407,70 +407,188 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
   return lam_new;
 }
 
-// Half path integral 1/2 int B dpsi of face i+1/2 of a two-phase state (the
-// Rusanov D- = D+ of NumFluxes.jl:35-59), for the faces with a volume-fraction
-// jump; other faces are left untouched (the step reads only jump faces).
-template <int FAST>
-__device__ __forceinline__ void face_path2(const MatDev* mat, const double* __restrict__ Q,
-                                           double* __restrict__ D, long long i, long long n,
+// Whether face i+1/2 (0 <= i < n - 1) of a two-phase state has a volume-
+// fraction jump: the faces whose path integrals are not zero.  The step's own
+// test (chunk_step) is the same expression.
+__device__ __forceinline__ bool face_jump2(const double* __restrict__ Q, long long i,
                                            long long pitch) {
-  const bool valid = i >= 0 && i < n - 1;
-  const long long ic = valid ? i : 0;
-  const double a0 = Q[ic], a0n = Q[ic + 1], a1 = Q[15 * pitch + ic], a1n = Q[15 * pitch + ic + 1];
-  const bool jump = valid && !((a0n - a0) == 0.0 && (a1n - a1) == 0.0);
-  if (!__any(jump)) return;
-  if (jump) {
-    double q[30], qn[30], d[30];
+  const double a0 = Q[i], a0n = Q[i + 1], a1 = Q[15 * pitch + i], a1n = Q[15 * pitch + i + 1];
+  return !(((a0n - a0) == 0.0) & ((a1n - a1) == 0.0));
+}
+
+// One quadrature node's share of a path integral (NumFluxes.jl:39-47, 97-107):
+// w b^p(psi) da_p in the rows of phase block p, where psi is the state at the
+// node, b^p the non-zero column of B's block p and da_p the volume-fraction
+// jump of the path.  The integral is the sum over the nodes.
+template <int FAST>
+__device__ __forceinline__ void bnode2(const MatDev* mat, const double* psi, double w, double da0,
+                                       double da1, double* out) {
+  BCol b;
+  bcol2<FAST>(mat, psi, b);
 #pragma unroll
-    for (int r = 0; r < 30; ++r) {
-      q[r] = Q[r * pitch + i];
-      qn[r] = Q[r * pitch + i + 1];
-    }
-    bint2<true, FAST>(mat, q, qn, d);
+  for (int p = 0; p < 2; ++p) {
+    const double wd = w * (p == 0 ? da0 : da1);
+    double* o = out + 15 * p;
+    o[0] = wd * b.uI0;
+    o[1] = 0.0;
+    o[2] = wd * b.sI[0];
+    o[3] = wd * b.sI[1];
+    o[4] = wd * b.sI[2];
+    o[5] = wd * b.sIu;
 #pragma unroll
-    for (int r = 0; r < 30; ++r) D[r * pitch + i] = 0.5 * d[r];
+    for (int r = 0; r < 9; ++r) o[6 + r] = wd * b.a[p][r];
   }
 }
 
-// Two phases, HLL: for the faces with a volume-fraction jump, the full D-, D+
-// (NumFluxes.jl:70-132) minus the path-free part cl J + clr dQ, cr J - clr dQ
-// (rows 0..29: D-, rows 30..59: D+ of face i+1/2).  C holds the stored sound
-// speeds (2 rows), as the step uses them.
+// Sum of v[30] over the 16 lanes of a lane group (every lane of the group active).
+__device__ __forceinline__ void group16_sum(double* v) {
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1)
+#pragma unroll
+    for (int r = 0; r < 30; ++r) v[r] += __shfl_xor(v[r], o, 16);
+}
+
+// v[g] and v[g + 15] for lane g of a group (g < 15), by selects: indexing a
+// register array with a lane-dependent index would move it to scratch memory.
+__device__ __forceinline__ void rows_of_lane(const double* v, int g, double& lo, double& hi) {
+  lo = 0.0;
+  hi = 0.0;
+#pragma unroll
+  for (int r = 0; r < 15; ++r) {
+    lo = g == r ? v[r] : lo;
+    hi = g == r ? v[r + 15] : hi;
+  }
+}
+
+// The jump faces' path terms are computed by a group of 16 lanes per face
+// (lane g of the group), one quadrature node per lane: the nodes evaluate the
+// same code on different data, which a wave runs in parallel.
+
+// Rusanov: half path integral 1/2 int B dpsi of the jump face i+1/2
+// (0 <= i < n - 1) of a two-phase state (D- = D+ of NumFluxes.jl:35-59),
+// Gauss-Lobatto nodes on lanes 0..5; only jump faces are written (the step
+// reads only those).
+template <int FAST>
+__device__ __forceinline__ void face_path2(const MatDev* mat, const double* __restrict__ Q,
+                                           double* __restrict__ D, long long i, long long pitch,
+                                           int g) {
+  double q[30], qn[30], d[30];
+#pragma unroll
+  for (int r = 0; r < 30; ++r) {
+    q[r] = Q[r * pitch + i];
+    qn[r] = Q[r * pitch + i + 1];
+    d[r] = 0.0;
+  }
+  if (g < 6) {
+    const double s = c_lo_s[g], oms = 1.0 - s;
+    double psi[30];
+#pragma unroll
+    for (int r = 0; r < 30; ++r) psi[r] = q[r] * oms + qn[r] * s;
+    bnode2<FAST>(mat, psi, c_lo_w[g], qn[0] - q[0], qn[15] - q[15], d);
+  }
+  group16_sum(d);
+  if (g < 15) {
+    double lo, hi;
+    rows_of_lane(d, g, lo, hi);
+    D[g * pitch + i] = 0.5 * lo;
+    D[(g + 15) * pitch + i] = 0.5 * hi;
+  }
+}
+
+// HLL: for the jump face i+1/2, D-, D+ (NumFluxes.jl:70-132) minus their
+// path-free part cl J + clr dQ, cr J - clr dQ, i.e. cl and cr times the two
+// half-path integrals through the HLL state (rows 0..29: D-, rows 30..59: D+
+// of face i+1/2).  C holds the stored sound speeds (2 rows), as the step uses
+// them.  Lanes 0..5: Gauss nodes of Q_l -> Q_r; lanes 0 and 1: the two cells'
+// fluxes, then one phase each of the mid-state eigenvalue range; lanes 0..5
+// and 6..11: Gauss nodes of Q_l -> Q_hll and Q_hll -> Q_r.
 template <int FAST>
 __device__ __forceinline__ void face_path2_hll(const MatDev* mat, const double* __restrict__ Q,
                                                const double* __restrict__ C,
-                                               double* __restrict__ D, long long i, long long n,
-                                               long long pitch) {
-  const bool valid = i >= 0 && i < n - 1;
-  const long long ic = valid ? i : 0;
-  const double a0 = Q[ic], a0n = Q[ic + 1], a1 = Q[15 * pitch + ic], a1n = Q[15 * pitch + ic + 1];
-  const bool jump = valid && !((a0n - a0) == 0.0 && (a1n - a1) == 0.0);
-  if (!__any(jump)) return;
-  if (jump) {
-    double q[30], qn[30], f[30], fn[30], dm[30], dp[30], cc[2], ccn[2];
+                                               double* __restrict__ D, long long i,
+                                               long long pitch, int g) {
+  double q[30], qn[30], bs[30];
 #pragma unroll
-    for (int r = 0; r < 30; ++r) {
-      q[r] = Q[r * pitch + i];
-      qn[r] = Q[r * pitch + i + 1];
-    }
+  for (int r = 0; r < 30; ++r) {
+    q[r] = Q[r * pitch + i];
+    qn[r] = Q[r * pitch + i + 1];
+    bs[r] = 0.0;
+  }
+  // B_int(Q_l, Q_r)
+  if (g < 6) {
+    const double s = c_gl_s[g], oms = 1.0 - s;
+    double psi[30];
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      cc[p] = C[p * pitch + i];
-      ccn[p] = C[p * pitch + i + 1];
-    }
-    double smin, smax, smin_n, smax_n;
+    for (int r = 0; r < 30; ++r) psi[r] = q[r] * oms + qn[r] * s;
+    bnode2<FAST>(mat, psi, c_gl_w[g], qn[0] - q[0], qn[15] - q[15], bs);
+  }
+  // F(Q_l) on lane 0, F(Q_r) on lane 1, with the speed bounds cell_flux forms
+  double fx[30], lo = 0.0, hi = 0.0;
+#pragma unroll
+  for (int r = 0; r < 30; ++r) fx[r] = 0.0;
+  if (g < 2) {
+    double st[30], cs[2];
+#pragma unroll
+    for (int r = 0; r < 30; ++r) st[r] = g == 0 ? q[r] : qn[r];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) cs[p] = C[p * pitch + i + g];
     int bad;
-    cell_flux<2, FAST>(mat, q, cc, f, smin, smax, bad);
-    cell_flux<2, FAST>(mat, qn, ccn, fn, smin_n, smax_n, bad);
-    HllCoef hc;
-    face_hll<2, FAST>(mat, q, qn, f, fn, smin, smax_n, dm, dp, &hc);
+    cell_flux<2, FAST>(mat, st, cs, fx, lo, hi, bad);
+  }
+  // eigenvalue range of the mid state (Q_l + Q_r) / 2, phase g on lane g
+  double mlo = 0.0, mhi = 0.0;
+  if (g < 2) {
+    double qm[15];
 #pragma unroll
-    for (int r = 0; r < 30; ++r) {
-      const double J = fn[r] - f[r];
-      const double dq = qn[r] - q[r];
-      D[r * pitch + i] = dm[r] - (hc.cl * J + hc.clr * dq);
-      D[(30 + r) * pitch + i] = dp[r] - (hc.cr * J - hc.clr * dq);
-    }
+    for (int r = 0; r < 15; ++r) qm[r] = 0.5 * ((g == 0 ? q[r] : q[15 + r]) + (g == 0 ? qn[r] : qn[15 + r]));
+    Thermo t;
+    thermo<false, FAST>(mat[g], qm[0], qm + 1, t);
+    const double c = sound_speed(mat[g], t);
+    mlo = t.u[0] - c;
+    mhi = t.u[0] + c;
+  }
+  group16_sum(bs);
+  const double smin_l = __shfl(lo, 0, 16), smax_r = __shfl(hi, 1, 16);
+  const double mn_m = fmin(__shfl(mlo, 0, 16), __shfl(mlo, 1, 16));
+  const double mx_m = fmax(__shfl(mhi, 0, 16), __shfl(mhi, 1, 16));
+  const HllCoef hc = hll_coef(mn_m, mx_m, smin_l, smax_r);
+  // Q_hll from J = B_int(Q_l, Q_r) + F(Q_r) - F(Q_l), as face_hll forms it
+  double qh[30];
+  double J0 = 0.0, J15 = 0.0;
+#pragma unroll
+  for (int r = 0; r < 30; ++r) {
+    const double fl = __shfl(fx[r], 0, 16), fr = __shfl(fx[r], 1, 16);
+    const double J = bs[r] + (fr - fl);
+    qh[r] = (qn[r] * hc.s_r - q[r] * hc.s_l - J) * hc.isd;
+    if (r == 0) J0 = J;
+    if (r == 15) J15 = J;
+  }
+  qh[0] = q[0] + (hc.s_r * (qn[0] - q[0]) - J0) * hc.isd;
+  qh[15] = q[15] + (hc.s_r * (qn[15] - q[15]) - J15) * hc.isd;
+  // B_int(Q_l, Q_hll) + B_int(Q_hll, Q_r)
+  double bb[30];
+#pragma unroll
+  for (int r = 0; r < 30; ++r) bb[r] = 0.0;
+  if (g < 12) {
+    const bool first = g < 6;
+    const int k = first ? g : g - 6;
+    const double s = c_gl_s[k], oms = 1.0 - s;
+    double psi[30];
+#pragma unroll
+    for (int r = 0; r < 30; ++r)
+      psi[r] = (first ? q[r] : qh[r]) * oms + (first ? qh[r] : qn[r]) * s;
+    const double da0 = first ? qh[0] - q[0] : qn[0] - qh[0];
+    const double da1 = first ? qh[15] - q[15] : qn[15] - qh[15];
+    bnode2<FAST>(mat, psi, c_gl_w[k], da0, da1, bb);
+  }
+  group16_sum(bb);
+  if (g < 15) {
+    double lo, hi;
+    rows_of_lane(bb, g, lo, hi);
+    D[g * pitch + i] = hc.cl * lo;
+    D[(30 + g) * pitch + i] = hc.cr * lo;
+    D[(g + 15) * pitch + i] = hc.cl * hi;
+    D[(45 + g) * pitch + i] = hc.cr * hi;
   }
 }
 
