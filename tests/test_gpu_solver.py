@@ -262,3 +262,43 @@ def test_relaxation_off_is_the_plain_step_and_conserves(ora):
         t0 = Q0[rows].sum()
         t1 = Q[rows].sum()
         assert abs(t1 - t0) <= 1e-12 * np.abs(Q0[rows]).sum(), rows
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "hll"])
+def test_two_phase_every_face_jumps(ora, flux):
+    """A tanh volume-fraction profile (main.jl:110-123) makes every face a
+    jump face, so the jump-face list of the path-term prepass is full."""
+    from hyperelasticsolver_amd import driver
+    print(f"[two-phase] tanh profile, every face a jump, flux={flux}", flush=True)
+    rng = np.random.default_rng(12)
+    mats = np.concatenate([MAT_CU, MAT_AL])
+    Qs = ora.prim2cons(2, mats, _rand_two_phase(rng, mats, 2))
+    nx, steps = 2048, 4
+    Q0 = np.asfortranarray(driver.initial_condition_tanh(_two_phase_eos(mats), Qs[:, 0], Qs[:, 1],
+                                                         nx, 0.5))
+    assert np.all(np.diff(Q0[0]) != 0.0)
+    Qo, to, so, dto = ora.run(2, mats, FK[flux], Q0, 0.6, 1.0 / nx, steps=steps)
+    Qg, r = _gpu_run(2, mats, flux, Q0, steps)
+    err = rel_err_per_component(Qg, Qo)
+    print(f"  max per-component rel err {err.max():.3e}", flush=True)
+    assert np.all(err <= TOL), err
+    np.testing.assert_allclose(r["dt"], dto, rtol=TOL, atol=0)
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "hll"])
+def test_two_phase_mixture_conserved_at_scale(ora, flux):
+    """2^16 cells, 5 pieces, equal end states: each phase's mass and the
+    mixture momentum and energy telescope to round-off (the path terms of
+    the two phases cancel in the mixture rows since a1 + a2 = 1)."""
+    print(f"[two-phase] conservation, 2^16 cells, flux={flux}", flush=True)
+    rng = np.random.default_rng(13)
+    mats = np.concatenate([MAT_CU, MAT_AL])
+    Qs = ora.prim2cons(2, mats, _rand_two_phase(rng, mats, 4))
+    nx = 1 << 16
+    Q0 = _piecewise(nx, Qs[:, [0, 1, 2, 3, 0]], np.array([0.2, 0.4, 0.6, 0.8]))
+    Qg, r = _gpu_run(2, mats, flux, Q0, 30)
+    assert r["steps"] == 30
+    for rows in ([1], [16], [2, 17], [3, 18], [4, 19], [5, 20]):
+        t0, t1 = Q0[rows].sum(), Qg[rows].sum()
+        assert abs(t1 - t0) <= 1e-12 * np.abs(Q0[rows]).sum(), (rows, t1 - t0)
+    assert np.all(np.isfinite(Qg))
