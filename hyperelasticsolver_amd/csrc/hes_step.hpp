@@ -231,7 +231,8 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     // is formed from the same two rounded halves by both of its cells, so the
     // two cells see bit-identical face fluxes.
     const double lam_c = fmax(fabs(smin), fabs(smax));  // maximum(abs.(eigvals))
-    const double hg = 0.5 * (1.0 / dtdx);                  // main.jl:50
+    // lambda = 1 / dtdx (main.jl:50), to about an ulp (dt itself stays exact)
+    const double hg = a.lxf ? 0.5 * rcp(dtdx) : 0.0;
     const double lam_r = lane_down_z(lam_c), lam_l = lane_up_z(lam_c);
     const double hr = a.lxf ? hg : 0.5 * fmax(lam_c, lam_r);
     const double hl = a.lxf ? hg : 0.5 * fmax(lam_l, lam_c);
@@ -251,7 +252,7 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     // volume-fraction jump.
     const double lam_c = fmax(fabs(smin), fabs(smax));  // maximum(abs.(eigvals))
     const double lam_n = fmax(fabs(lane_down(smin)), fabs(lane_down(smax)));
-    const double hl = a.lxf ? 0.5 * (1.0 / dtdx) : 0.5 * fmax(lam_c, lam_n);  // main.jl:47-50
+    const double hl = a.lxf ? 0.5 * rcp(dtdx) : 0.5 * fmax(lam_c, lam_n);  // main.jl:47-50
     // the shifts run with every lane active (not under the lane-dependent
     // conditions), so each lane sees its right neighbour's volume fractions
     const double a0n = lane_down(q[0]), a1n = lane_down(q[15]);
@@ -363,7 +364,7 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
     double lam_c = fmax(fabs(smin), fabs(smax));       // maximum(abs.(eigvals))
     double lam_n = fmax(fabs(smin_n), fabs(smax_n));
     double cons[NC], dmp[NC];
-    face_rusanov<NPH, FAST>(a.mat, q, qn, f, fn, a.lxf ? 1.0 / dtdx : fmax(lam_c, lam_n), cons,
+    face_rusanov<NPH, FAST>(a.mat, q, qn, f, fn, a.lxf ? rcp(dtdx) : fmax(lam_c, lam_n), cons,
                             dmp);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
