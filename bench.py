@@ -53,6 +53,30 @@ def time_solver(eos, Qs, ifc, nx, flux, steps, warmup):
     return ms
 
 
+def host_transfer(eos, Qs, ifc, nx):
+    """Upload and download of the headline state between host memory (Julia's
+    ncomp x n column-major layout) and the device layout: what a caller that
+    hands over host arrays pays once per run, outside `value`."""
+    from hyperelasticsolver_amd.solver import Solver
+    x = (np.arange(nx) + 0.5) / nx
+    Q0 = np.asfortranarray(Qs[:, np.searchsorted(ifc, x, side="left")])
+    up, down = [], []
+    with Solver(eos, nx, flux="rusanov", cfl=0.6) as s:
+        for _ in range(3):
+            t0 = time.perf_counter()
+            s.set_state(Q0)
+            t1 = time.perf_counter()
+            Q = s.get_state()
+            t2 = time.perf_counter()
+            up.append(t1 - t0)
+            down.append(t2 - t1)
+    if not np.array_equal(Q, Q0):
+        raise RuntimeError("state changed in an upload/download round trip")
+    nbytes = Q0.nbytes
+    return {"state_bytes": nbytes, "upload_ms": min(up) * 1e3, "download_ms": min(down) * 1e3,
+            "upload_gbs": nbytes / min(up) / 1e9, "download_gbs": nbytes / min(down) / 1e9}
+
+
 def cpu_baseline(eos, Qs, ifc, flux_kind, threads):
     """The oracle (C++ restatement of the reference algorithm) on host cores,
     on a bounded sample of the same workload: 2^17 cells, 64 steps (about
@@ -242,14 +266,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="only the headline line (no HLL, batched or adjoint lines)")
-    ap.add_argument("--parts", default="hll,two_phase,batched,adjoint",
-                    help="extra lines to add: any of hll, two_phase, batched, adjoint")
+    ap.add_argument("--parts", default="hll,two_phase,transfer,batched,adjoint",
+                    help="extra lines to add: any of hll, two_phase, transfer, batched, adjoint")
     ap.add_argument("--adjoint-modes", default=",".join(ADJOINT_MODES))
     ap.add_argument("--batched-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU slab "
-                                   "decomposition is a design (see DESIGN.md), not measured",
+        print(json.dumps({"error": "this benchmark runs on one GPU only; the multi-GPU slab "
+                                   "decomposition is a design, not measured",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
@@ -294,6 +318,8 @@ def main():
                       "hbm_frac": BYTES_PER_CELL_1PH * v2 / 1e9 / HBM_PEAK_GBS}
     if "two_phase" in parts:
         out["two_phase"] = time_two_phase()
+    if "transfer" in parts:
+        out["host_transfer"] = host_transfer(eos, Qs, ifc, nx)
     if "batched" in parts:
         out["batched"] = time_batched(args.batched_steps)
     if "adjoint" in parts:

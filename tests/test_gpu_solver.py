@@ -125,7 +125,7 @@ def test_t_end_stops_like_reference(ora):
 
 @pytest.mark.parametrize("nph", [1, 2])
 def test_uniform_state_is_bit_invariant(ora, nph):
-    """F(Q) - F(Q) = 0 exactly: a uniform state never changes (SURVEY known answer)."""
+    """F(Q) - F(Q) = 0 exactly: a uniform state never changes (a known answer of the scheme)."""
     print(f"[uniform] nph={nph}", flush=True)
     mats, _, Qs, _ = _problem(ora, nph, 10, 1, seed=3)
     Q0 = np.asfortranarray(np.repeat(Qs[:, :1], 130, axis=1))
