@@ -269,6 +269,31 @@ def test_flux_at_rest_is_minus_stress(ora):
     assert F[0] == 0 and F[4] == 0 and np.all(F[5:] == 0)
 
 
+def test_flux_jacobian_spectrum_is_the_characteristic_speeds(ora):
+    """The 1D system Q_t + F(Q)_x = 0 is hyperbolic with characteristic speeds
+    u1 +- c_k (c_k^2 the eigenvalues of the acoustic tensor), u1 for the
+    advected components and 0 for the three first-row entries of rho A, whose
+    flux vanishes identically.  So the eigenvalues of dF/dQ (central
+    differences of flux, HyperelasticityMPh.jl:146-175) must be exactly the
+    six values get_eigvals returns (HyperelasticityMPh.jl:252-266), u1 four
+    times and 0 four times: a dropped or misplaced flux term moves them."""
+    rng = np.random.default_rng(31)
+    for _ in range(4):
+        Q = ora.prim2cons(1, MAT, _rand_prims(rng, 1, MAT))
+        n = Q.size
+        J = np.zeros((n, n))
+        for k in range(n):
+            h = 1e-6 * max(abs(Q[k]), 1e-3)
+            Qp, Qm = Q.copy(), Q.copy()
+            Qp[k] += h
+            Qm[k] -= h
+            J[:, k] = (ora.flux(1, MAT, Qp) - ora.flux(1, MAT, Qm)) / (2 * h)
+        got = np.sort(np.linalg.eigvals(J).real)
+        u1 = ora.cons2prim(1, MAT, Q)[1]
+        want = np.sort(np.concatenate([ora.get_eigvals(1, MAT, Q), [u1] * 4, [0.0] * 4]))
+        assert np.abs(got - want).max() <= 1e-7 * np.abs(want).max()
+
+
 def test_noncons_structure(ora):
     rng = np.random.default_rng(13)
     Q = ora.prim2cons(2, MAT2, _rand_prims(rng, 2, MAT2))
