@@ -294,6 +294,57 @@ def test_flux_jacobian_spectrum_is_the_characteristic_speeds(ora):
         assert np.abs(got - want).max() <= 1e-7 * np.abs(want).max()
 
 
+def test_interface_exchange_of_one_rusanov_step(ora):
+    """Two pieces whose phases differ only in volume fraction: along the path
+    psi(s) = Q_l (1-s) + Q_r s every primitive is constant, so B(psi) is, and
+    one Rusanov step (D- = D+ = 1/2 int B dpsi, main.jl:59) changes the sums
+    over all cells by exactly -dt/dx [F(Q_r) - F(Q_l) + B da].  B's column
+    (HyperelasticityMPh.jl:204-226, omega = 0, k = 1/2, beta = 0) is built here
+    from the separately pinned stress and temperature: u_I = (u1 + u2)/2,
+    sigma_I = (th2 sigma1 + th1 sigma2)/(th1 + th2), sigma_I(:,1).u_I, and the
+    distortion rows rho A_1c u + rho (A^T (u_I - u))_c.  Swapping the
+    temperature weights, or taking u_I = u1 or sigma_I = sigma1, misses by a
+    factor of 2 or more."""
+    rng = np.random.default_rng(41)
+    P = _rand_prims(rng, 2, MAT2)
+    PL, PR = P.copy(), P.copy()
+    PL[0], PL[15] = 0.3, 0.7
+    PR[0], PR[15] = 0.8, 0.2
+    QL, QR = ora.prim2cons(2, MAT2, PL), ora.prim2cons(2, MAT2, PR)
+    nx = 20
+    Q0 = np.asfortranarray(np.stack([QL if i < nx // 2 else QR for i in range(nx)], axis=1))
+    Q1, t, s, dt = ora.run(2, MAT2, 0, Q0, 0.6, 1.0 / nx, steps=1)
+    dtdx = dt[0] * nx
+    pr = ora.cons2prim(2, MAT2, QL)
+    sig, th, u, rho, A = [], [], [], [], []
+    for p in range(2):
+        m = MAT2[8 * p:8 * p + 8]
+        b = pr[15 * p:15 * p + 15]
+        sig.append(mat(ora.stress(m, b[5], b[6:15])))
+        th.append(ora.temperature(m, b[5], ora.finger(b[6:15])))
+        u.append(b[2:5])
+        rho.append(b[1])
+        A.append(mat(b[6:15]))
+    uI = 0.5 * u[0] + 0.5 * u[1]
+    sI = (th[1] * sig[0] + th[0] * sig[1]) / (th[0] + th[1])
+    FL, FR = ora.flux(2, MAT2, QL), ora.flux(2, MAT2, QR)
+    dS = Q1.sum(axis=1) - Q0.sum(axis=1)
+    for p in range(2):
+        da = PR[15 * p] - PL[15 * p]
+        Bc = np.zeros(15)
+        Bc[0] = uI[0]
+        Bc[2:5] = sI[:, 0]
+        Bc[5] = sI[:, 0] @ uI
+        atdv = A[p].T @ (uI - u[p])
+        for c in range(3):
+            Bc[6 + 3 * c:9 + 3 * c] = rho[p] * A[p][0, c] * u[p]
+            Bc[6 + 3 * c] += rho[p] * atdv[c]
+        o = 15 * p
+        pred = -dtdx * (FR[o:o + 15] - FL[o:o + 15] + Bc * da)
+        scale = np.abs(pred) + 1e-12 * np.abs(pred).max()
+        assert np.all(np.abs(dS[o:o + 15] - pred) <= 1e-11 * scale), (p, dS[o:o + 15] - pred)
+
+
 def test_noncons_structure(ora):
     rng = np.random.default_rng(13)
     Q = ora.prim2cons(2, MAT2, _rand_prims(rng, 2, MAT2))
