@@ -138,20 +138,27 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : HES_STEP2_
 
 // Two phases: the faces with a volume-fraction jump of the step's input state,
 // appended to a list (face index i of face i+1/2) whose length is counted in
-// jcnt[step & 1]; the other slot is cleared here for the next step.  One face
+// jcnt[step & 1]; the other slot is cleared here for the next step.  A step
+// that is a no-op (t >= t_end) also clears its own slot: the host does not
+// count it, so the next run starts at the same step index and slot.  One face
 // per thread, a ballot and one atomic per wave.
 __global__ __launch_bounds__(BLOCK) void k_mark2(StepArgs a) {
   const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.jcnt[(a.step + 1) & 1] = 0ull;
-  if (!(a.tring[a.step & 1] < a.t_end)) return;  // the step is a no-op too
+  const bool live = a.tring[a.step & 1] < a.t_end;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.jcnt[(a.step + 1) & 1] = 0ull;
+    if (!live) a.jcnt[a.step & 1] = 0ull;
+  }
+  if (!live) return;  // the step is a no-op too
   const bool jump = i < a.n - 1 && face_jump2(a.Qin, i, a.pitch);
   const unsigned long long m = __ballot(jump);
   if (m == 0ull) return;
   unsigned long long base = 0ull;
   if (lane == 0) base = atomicAdd(a.jcnt + (a.step & 1), (unsigned long long)__popcll(m));
   base = __shfl(base, 0);
-  if (jump) a.jlist[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
+  const unsigned long long pos = base + __popcll(m & ((1ull << lane) - 1ull));
+  if (jump && pos < (unsigned long long)a.n) a.jlist[pos] = i;  // at most n - 1 faces
 }
 
 // The path terms of the listed jump faces over a fixed grid: a group of 16
@@ -160,7 +167,7 @@ __global__ __launch_bounds__(BLOCK) void k_mark2(StepArgs a) {
 template <int FAST>
 __global__ __launch_bounds__(64) void k_path2(StepArgs a) {
   if (!(a.tring[a.step & 1] < a.t_end)) return;
-  const unsigned long long cnt = a.jcnt[a.step & 1];
+  const unsigned long long cnt = min(a.jcnt[a.step & 1], (unsigned long long)a.n - 1ull);
   const unsigned long long stride = (unsigned long long)gridDim.x * 4;
   for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 4; j < cnt;
        j += stride)
@@ -170,7 +177,7 @@ __global__ __launch_bounds__(64) void k_path2(StepArgs a) {
 template <int FAST>
 __global__ __launch_bounds__(64) void k_path2_hll(StepArgs a) {
   if (!(a.tring[a.step & 1] < a.t_end)) return;
-  const unsigned long long cnt = a.jcnt[a.step & 1];
+  const unsigned long long cnt = min(a.jcnt[a.step & 1], (unsigned long long)a.n - 1ull);
   const unsigned long long stride = (unsigned long long)gridDim.x * 4;
   for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 4; j < cnt;
        j += stride)

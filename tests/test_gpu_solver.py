@@ -393,3 +393,37 @@ def test_volume_fractions_keep_summing_to_one(ora):
         Qg, r = _gpu_run(2, mats, flux, Q0, 40)
         assert r["steps"] == 40
         assert np.abs(Qg[0] + Qg[15] - 1.0).max() <= 1e-13, flux
+
+
+@pytest.mark.parametrize("live,use_graph", [(12, False), (59, True)])
+def test_two_phase_run_resumed_after_one_no_op_step(ora, live, use_graph):
+    """A run whose last launched step is a no-op (t >= t_end) and a run that
+    continues it give the uninterrupted run bit for bit.  On a tanh profile
+    every face is a jump face, so the jump-face count that the no-op step
+    leaves behind must be cleared, or the next step would list faces twice.
+    The runner launches full chunks of 60 steps as one graph, so with 59 live
+    steps the no-op is the last step of a captured graph."""
+    from hyperelasticsolver_amd import driver
+    from hyperelasticsolver_amd.solver import Solver
+    print(f"[two-phase] resume after one no-op step, {live} live steps, graph={use_graph}",
+          flush=True)
+    rng = np.random.default_rng(15)
+    mats = np.concatenate([MAT_CU, MAT_AL])
+    Qs = ora.prim2cons(2, mats, _rand_two_phase(rng, mats, 2))
+    eos = _two_phase_eos(mats)
+    nx = 512
+    Q0 = np.asfortranarray(driver.initial_condition_tanh(eos, Qs[:, 0], Qs[:, 1], nx, 0.5))
+    with Solver(eos, nx, flux="hll", cfl=0.6) as s:
+        s.set_state(Q0)
+        r = s.run(steps=live, use_graph=False)
+        T1 = r["t"]                       # reached after exactly `live` steps
+        s.set_state(Q0)
+        r1 = s.run(steps=live + 1, t_end=T1, use_graph=use_graph)   # the last launch is a no-op
+        assert r1["steps"] == live and r1["t"] == T1
+        r2 = s.run(steps=8, t_end=math.inf, use_graph=False)
+        Qa = s.get_state()
+        s.set_state(Q0)
+        r3 = s.run(steps=live + 8, t_end=math.inf, use_graph=False)
+        Qb = s.get_state()
+    assert r2["steps"] == 8 and r3["steps"] == live + 8 and r2["t"] == r3["t"]
+    assert np.array_equal(Qa, Qb)
