@@ -7,7 +7,7 @@ S = 1e-3, rho = rho0/det A) separated by 64 seeded interfaces, Barton2009
 copper constants, CFL 0.6, Rusanov flux.  A "step" is one time step over all
 cells (one launch of the fused step kernel).  The state is resident in HBM
 before the timed region; HIP events on the solver's stream bracket exactly
-the K timed launches.
+the K timed steps (replayed from one graph captured beforehand).
 
 Prints one JSON line.
 """
@@ -195,7 +195,8 @@ ADJOINT_MODES = ("graph_sparse", "graph_dense", "eager_sparse", "eager_dense")
 
 def time_adjoint(n=1 << 18, steps=2000, every=50, modes=ADJOINT_MODES):
     """(forward + adjoint) / forward: the plain forward is the solver's step
-    loop on the same problem (one launch per step); the adjoint run is its
+    loop on the same problem (its steps replayed from one captured graph, as
+    the adjoint's graph modes replay theirs); the adjoint run is its
     checkpointed forward pass plus the reverse pass (recompute + reverse
     steps).  Reported for each launch mode, with zero-tile skipping (the
     default) and without it (every tile of every reverse step evaluated)."""

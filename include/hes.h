@@ -90,8 +90,10 @@ int hes_solver_run(hes_solver* h, long long max_steps, double t_end, int use_gra
  * tau = INFINITY switches it off (the default). */
 int hes_solver_set_relaxation(hes_solver* h, double tau);
 
-/* Time `reps` back-to-back launches of the step kernel (and nothing else) with
- * HIP events on the solver stream; the state advances by reps steps. */
+/* Time `reps` back-to-back steps (the step kernels and nothing else) with HIP
+ * events on the solver stream; the state advances by reps steps.  The steps
+ * are captured into one graph before the first event (as hes_solver_run
+ * replays them), so the timed region holds only the graph's launch. */
 int hes_solver_time_steps(hes_solver* h, long long reps, float* elapsed_ms);
 
 /* Measured HBM bandwidth of a plain device copy of `bytes` bytes (read plus
