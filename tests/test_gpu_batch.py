@@ -175,3 +175,32 @@ def test_autotune_on_reused_memory_matches_clean_run(ora):
     b.run(6, mode="eager")
     assert np.array_equal(b.get_state(), clean)
     b.close()
+
+
+def test_batch_general_exponents_match_oracle(ora):
+    """A batch with one problem of general exponents (main.jl:133 constants)
+    runs the general-exponent kernels for all its problems."""
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    from hyperelasticsolver_amd.batched import Batch
+    from hyperelasticsolver_amd.solver import materials_array
+    print("[batch] general exponents vs oracle", flush=True)
+    rng = np.random.default_rng(21)
+    eos = [Barton2009(), Barton2009(_c0=6.22, _cv=9.0e-4, _b0=3.16, _beta=3.577, _gamma=2.088),
+           Barton2009(_rho0=2.7, _b0=3.1)]
+    ncells = [90, 130, 75]
+    sts, ifs = [], []
+    for e in eos:
+        Qs, ifc = _problem_states(ora, rng, e, 1)
+        sts.append(Qs)
+        ifs.append(ifc)
+    with Batch(eos, ncells, flux="hll") as b:
+        b.set_piecewise(sts, ifs)
+        Q0 = b.get_state()
+        b.run(10, mode="eager")
+        Qb = b.get_state()
+        offs = b.offsets
+    for p, n in enumerate(ncells):
+        Qo, _, _, _ = ora.run(1, materials_array(eos[p]), 1, Q0[:, offs[p]:offs[p + 1]], 0.6,
+                              1.0 / n, steps=10)
+        err = rel_err_per_component(Qb[:, offs[p]:offs[p + 1]], Qo)
+        assert np.all(err <= TOL), (p, err.max())

@@ -8,6 +8,8 @@ pytestmark = pytest.mark.gpu
 
 MAT1 = np.array([8.93, 4.6, 3.9e-4, 300.0, 2.1, 1.0, 3.0, 2.0])
 MAT2 = np.array([2.7, 4.6, 3.9e-4, 300.0, 3.1, 1.0, 3.0, 2.0])
+# the alternative constants of main.jl:133: general exponents (beta, gamma not 3, 2)
+MAT_ALT = np.array([8.93, 6.22, 9.0e-4, 300.0, 3.16, 1.0, 3.577, 2.088])
 NPTS = 48
 
 
@@ -55,31 +57,34 @@ def test_strains(ora):
         _close(invariants(G[:, k]), ora.invariants(G[:, k]), 1e-13, "invariants")
 
 
-def test_equations_of_state(ora):
+@pytest.mark.parametrize("which", ["default", "general exponents"])
+def test_equations_of_state(ora, which):
     from hyperelasticsolver_amd import EquationsOfState as E
     from hyperelasticsolver_amd.Strains import finger
-    print("[pointwise] EquationsOfState", flush=True)
-    eos = _eos()[0]
+    print(f"[pointwise] EquationsOfState, {which}", flush=True)
+    m = MAT1 if which == "default" else MAT_ALT
+    eos = E.Barton2009(_rho0=m[0], _c0=m[1], _cv=m[2], _t0=m[3], _b0=m[4], _alpha=m[5],
+                       _beta=m[6], _gamma=m[7])
     rng = np.random.default_rng(2)
     for _ in range(NPTS):
         A = _A(rng)
         S = rng.uniform(0, 2e-3)
         G = finger(A)
         e = E.energy(eos, S, G)
-        assert e == pytest.approx(ora.energy(MAT1, S, G), rel=1e-13)
-        assert E.entropy(eos, e, G) == pytest.approx(ora.entropy(MAT1, e, G), rel=1e-10, abs=1e-15)
-        assert E.temperature(eos, S, G) == pytest.approx(ora.temperature(MAT1, S, G), rel=1e-13)
-        _close(E.stress(eos, S, A), ora.stress(MAT1, S, A), 1e-11, "stress")
-        den = MAT1[0] / np.linalg.det(A.reshape(3, 3, order="F"))
-        _close(E.stress(eos, den, e, A), ora.stress_den(MAT1, den, e, A), 1e-10, "stress_den")
+        assert e == pytest.approx(ora.energy(m, S, G), rel=1e-13)
+        assert E.entropy(eos, e, G) == pytest.approx(ora.entropy(m, e, G), rel=1e-10, abs=1e-15)
+        assert E.temperature(eos, S, G) == pytest.approx(ora.temperature(m, S, G), rel=1e-13)
+        _close(E.stress(eos, S, A), ora.stress(m, S, A), 1e-11, "stress")
+        den = m[0] / np.linalg.det(A.reshape(3, 3, order="F"))
+        _close(E.stress(eos, den, e, A), ora.stress_den(m, den, e, A), 1e-10, "stress_den")
         n = rng.standard_normal(3)
         n /= np.linalg.norm(n)
-        _close(E.acoustic(eos, S, A, n), ora.acoustic(MAT1, S, A, n), 1e-10, "acoustic")
+        _close(E.acoustic(eos, S, A, n), ora.acoustic(m, S, A, n), 1e-10, "acoustic")
         Pv = np.concatenate([[1.0, den], rng.uniform(-0.5, 0.5, 3), [S], A])
-        _close(E.acoustic(eos, Pv, n), ora.acoustic_prim(MAT1, Pv, n), 1e-10, "acoustic_prim")
+        _close(E.acoustic(eos, Pv, n), ora.acoustic_prim(m, Pv, n), 1e-10, "acoustic_prim")
     # the entropy clamp (EquationsOfState.jl:152-154): far below the cold energy
     G = finger(np.eye(3).reshape(-1) * 0.9)
-    assert E.entropy(eos, -10.0, G) == pytest.approx(ora.entropy(MAT1, -10.0, G), rel=1e-13)
+    assert E.entropy(eos, -10.0, G) == pytest.approx(ora.entropy(m, -10.0, G), rel=1e-13)
 
 
 def test_multiphase_module(ora):

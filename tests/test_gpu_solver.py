@@ -100,6 +100,36 @@ def test_step_parity(ora, nph, flux, nx, steps, nstates):
     assert abs(r["t"] - to) <= TOL * to
 
 
+MAT_ALT = np.array([8.93, 6.22, 9.0e-4, 300.0, 3.16, 1.0, 3.577, 2.088])   # main.jl:133
+MAT_ALT2 = np.array([2.7, 6.22, 9.0e-4, 300.0, 3.1, 1.0, 3.577, 2.088])
+
+
+@pytest.mark.parametrize("nph,flux", [(1, "rusanov"), (1, "hll"), (1, "lxf"),
+                                      (2, "rusanov"), (2, "hll"), (2, "lxf")])
+def test_step_parity_general_exponents(ora, nph, flux):
+    """The alternative constants of main.jl:133 (beta = 3.577, gamma = 2.088)
+    take the general-exponent kernels (powers through log/exp) instead of the
+    rho/rho0 products of beta = 3, gamma = 2."""
+    print(f"[parity] general exponents, nph={nph} flux={flux}", flush=True)
+    rng = np.random.default_rng(200 + nph)
+    if nph == 1:
+        mats = MAT_ALT.copy()
+        Ps = _rand_phase(rng, mats, 4)
+        nx, steps = 300, 40
+    else:
+        mats = np.concatenate([MAT_ALT, MAT_ALT2])
+        Ps = _rand_two_phase(rng, mats, 3)
+        nx, steps = 96, 12
+    Qs = ora.prim2cons(nph, mats, Ps)
+    Q0 = _piecewise(nx, Qs, np.sort(rng.uniform(0.1, 0.9, Qs.shape[1] - 1)))
+    Qo, to, so, dto = ora.run(nph, mats, FK[flux], Q0, 0.6, 1.0 / nx, steps=steps)
+    Qg, r = _gpu_run(nph, mats, flux, Q0, steps)
+    err = rel_err_per_component(Qg, Qo)
+    print(f"  max per-component rel err {err.max():.3e}", flush=True)
+    assert np.all(err <= TOL), err
+    np.testing.assert_allclose(r["dt"], dto, rtol=TOL, atol=0)
+
+
 def test_graph_equals_eager(ora):
     """The hipGraph replay path gives bit-identical states to eager launches."""
     print("[graph] single-phase hll nx=500 steps=125", flush=True)
