@@ -116,8 +116,8 @@ __device__ __forceinline__ void step_epilogue(const StepArgs& a, double lam_new,
   }
 }
 
-// One chunk per wave.
-template <int NPH, int FLUX, int FAST>
+// One chunk per wave.  PIN: see chunk_step (single phase, Rusanov, large grids).
+template <int NPH, int FLUX, int FAST, int PIN = 0>
 __global__ __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : HES_STEP2_WAVES_PER_EU))) void k_step(StepArgs a) {
   constexpr int NC = NComp<NPH>::value;
@@ -132,7 +132,7 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : HES_STEP2_
   load_chunk<NPH>(ctx, i, q, cc);
   const double lmax = step_lmax(a);
   constexpr bool PB = NPH == 2;  // path integrals from k_path2 / k_path2_hll
-  double lam = chunk_step<NPH, FLUX, FAST, PB>(ctx, i, q, cc, lmax);
+  double lam = chunk_step<NPH, FLUX, FAST, PB, PIN != 0>(ctx, i, q, cc, lmax);
   step_epilogue(a, lam, tcur, lmax);
 }
 
@@ -509,6 +509,16 @@ void launch_step_t(const StepArgs& a, long long nchunks, bool fast, hipStream_t 
     } else {
       if (fast) hipLaunchKernelGGL(k_path2_hll<1>, dim3(nbl), dim3(64), 0, s, a);
       else hipLaunchKernelGGL(k_path2_hll<0>, dim3(nbl), dim3(64), 0, s, a);
+    }
+  }
+  if constexpr (NPH == 1 && FLUX == FLUX_RUSANOV) {
+    // single phase, Rusanov, from 2^21 cells: the pinned schedule (chunk_step)
+    if (a.n >= (1LL << 21)) {
+      if (fast)
+        hipLaunchKernelGGL((k_step<NPH, FLUX, 1, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_step<NPH, FLUX, 0, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
+      return;
     }
   }
   if (fast)

@@ -167,7 +167,12 @@ __device__ __forceinline__ void relax_velocity2(const MatDev* mat, double* q, do
   q[20] += 0.5 * m[1] * (kn[1] - ke[1]) + 0.5 * dk;
 }
 
-template <int NPH, int FLUX, int FAST, bool PATHBUF = false>
+// PIN (single phase, Rusanov): each updated component is pinned where it is
+// computed (see pin): 110 instead of 160 VGPRs, four waves per SIMD.  It
+// pays on large grids (2^22 cells: 203 -> 200 us) and costs on small ones,
+// whose few waves per SIMD want the unpinned schedule's instruction-level
+// parallelism (2^18 cells: 19.6 -> 20.3 us).
+template <int NPH, int FLUX, int FAST, bool PATHBUF = false, bool PIN = false>
 __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, const double* q,
                                              const double* cc, double lmax,
                                              double* lam_cur = nullptr) {
@@ -243,6 +248,7 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
       double Fr = fa + lane_down_z(fb);
       double Fl = lane_up_z(fa) + fb;
       upd[r] = q[r] - dtdx * (Fr - Fl);  // main.jl:59 with zero path integrals
+      if (PIN) pin(upd[r]);
     }
   } else if (FLUX == FLUX_RUSANOV && NPH == 2 && PATHBUF) {
     // Two phases with the half path integrals precomputed (k_path2): each
