@@ -63,7 +63,7 @@ __device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double 
   if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
 }
 
-template <int NPH, int FLUX, int FAST, int BLK>
+template <int NPH, int FLUX, int FAST, int BLK, int PIN = 0>
 __global__ __launch_bounds__(BLK)
 __attribute__((amdgpu_waves_per_eu(NPH == 1 ? 3 : 1))) void k_batch_step(BatchArgs a) {
   constexpr int NC = NComp<NPH>::value;
@@ -88,7 +88,7 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? 3 : 1))) void k_batch_step(BatchAr
     return;
   }
   const double lmax = __longlong_as_double(*(const long long*)(a.slots + 3 * p + (a.step % 3)));
-  double lam = chunk_step<NPH, FLUX, FAST>(ctx, i, q, cc, lmax);
+  double lam = chunk_step<NPH, FLUX, FAST, false, PIN != 0>(ctx, i, q, cc, lmax);
   lam = wave_max(lam);
   if (lane == 0) {
     atomic_max_pos(a.slots + 3 * p + ((a.step + 1) % 3), lam);
@@ -237,6 +237,18 @@ template <int NPH, int FLUX, int FAST>
 void launch_batch_t(const BatchArgs& a, int block, hipStream_t st) {
   int nt = a.tile_end - a.tile_begin;
   if (nt <= 0) return;
+  if constexpr (NPH == 1) {
+    // launches over 2^21 cells and more (a whole batch; not the one-problem
+    // launches): the pinned schedule of chunk_step, four waves per SIMD
+    if ((long long)nt * TILE >= (1LL << 21)) {
+      if (block == 64)
+        hipLaunchKernelGGL((k_batch_step<NPH, FLUX, FAST, 64, 1>), dim3((unsigned)nt), dim3(64), 0, st, a);
+      else
+        hipLaunchKernelGGL((k_batch_step<NPH, FLUX, FAST, 256, 1>), dim3((unsigned)((nt + 3) / 4)),
+                           dim3(256), 0, st, a);
+      return;
+    }
+  }
   if (block == 64)
     hipLaunchKernelGGL((k_batch_step<NPH, FLUX, FAST, 64>), dim3((unsigned)nt), dim3(64), 0, st, a);
   else

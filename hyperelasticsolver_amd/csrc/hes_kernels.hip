@@ -511,8 +511,8 @@ void launch_step_t(const StepArgs& a, long long nchunks, bool fast, hipStream_t 
       else hipLaunchKernelGGL(k_path2_hll<0>, dim3(nbl), dim3(64), 0, s, a);
     }
   }
-  if constexpr (NPH == 1 && FLUX == FLUX_RUSANOV) {
-    // single phase, Rusanov, from 2^21 cells: the pinned schedule (chunk_step)
+  if constexpr (NPH == 1) {
+    // single phase from 2^21 cells: the pinned schedule (chunk_step)
     if (a.n >= (1LL << 21)) {
       if (fast)
         hipLaunchKernelGGL((k_step<NPH, FLUX, 1, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);

@@ -167,11 +167,12 @@ __device__ __forceinline__ void relax_velocity2(const MatDev* mat, double* q, do
   q[20] += 0.5 * m[1] * (kn[1] - ke[1]) + 0.5 * dk;
 }
 
-// PIN (single phase, Rusanov): each updated component is pinned where it is
-// computed (see pin): 110 instead of 160 VGPRs, four waves per SIMD.  It
-// pays on large grids (2^22 cells: 203 -> 200 us) and costs on small ones,
-// whose few waves per SIMD want the unpinned schedule's instruction-level
-// parallelism (2^18 cells: 19.6 -> 20.3 us).
+// PIN (single phase): each updated component is pinned where it is computed
+// (see pin): Rusanov 102 instead of 160 VGPRs, HLL 106 instead of 136, four
+// waves per SIMD instead of three.  It pays on large grids (2^22 cells:
+// Rusanov 203 -> 200 us, HLL 242 -> 229 us) and costs on small ones, whose
+// few waves per SIMD want the unpinned schedule's instruction-level
+// parallelism (2^18 cells, Rusanov: 19.6 -> 20.3 us).
 template <int NPH, int FLUX, int FAST, bool PATHBUF = false, bool PIN = false>
 __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, const double* q,
                                              const double* cc, double lmax,
@@ -316,7 +317,7 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
       const double dm = hc.cl * J + hc.clr * dq;
       const double dp = hc.cr * J - hc.clr * dq;
       upd[r] = q[r] - dtdx * (dm + lane_up(dp));  // conservative part is zero (NumFluxes.jl:82)
-      if (PHASED) pin(upd[r]);
+      if (PHASED || PIN) pin(upd[r]);
     };
     if (PHASED) {
 #pragma unroll

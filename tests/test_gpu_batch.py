@@ -84,6 +84,35 @@ def test_batch_equals_single_problem_runs(ora, flux):
         assert np.array_equal(Qb[:, b.offsets[p]:b.offsets[p + 1]], Qs), f"problem {p}"
 
 
+@pytest.mark.parametrize("flux,fk", [("rusanov", 0), ("hll", 1)])
+def test_batch_over_two_to_the_21_cells(ora, flux, fk):
+    """A whole-batch launch over 2^21 cells or more takes the pinned schedule
+    (four waves per SIMD); one-problem launches (mode "loop") take the other.
+    Both must give the same bits, and the oracle the same states to 1e-10."""
+    from hyperelasticsolver_amd.solver import materials_array
+    rng = np.random.default_rng(21)
+    ncells = [int(n) for n in rng.integers(40000, 52000, 48)]
+    assert sum((n + 61) // 62 for n in ncells) * 62 >= 1 << 21
+    steps = 8
+    print(f"[batch] over 2^21 cells ({sum(ncells)}), flux={flux}", flush=True)
+    res = []
+    for mode in ("graph", "loop"):
+        b, eos, sts, ifs = _build(ora, ncells, flux=flux, seed=4)
+        Q0 = b.get_state()
+        b.run(steps, mode=mode)
+        res.append((b.get_state(), b.times()))
+        offs = list(b.offsets)
+        b.close()
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1][0], res[1][1][0])
+    for p in (0, 17, 47):
+        n = ncells[p]
+        Qo, _, so, _ = ora.run(1, materials_array(eos[p]), fk, Q0[:, offs[p]:offs[p + 1]], 0.6,
+                               1.0 / n, steps=steps)
+        err = rel_err_per_component(res[0][0][:, offs[p]:offs[p + 1]], Qo)
+        assert so == steps and np.all(err <= TOL), (p, err.max())
+
+
 def test_batch_matches_oracle(ora):
     from hyperelasticsolver_amd.solver import materials_array
     print("[batch] vs oracle", flush=True)

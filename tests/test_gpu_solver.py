@@ -196,6 +196,31 @@ def test_parity_many_blocks(ora, flux):
     np.testing.assert_allclose(r["dt"], dto, rtol=TOL, atol=0)
 
 
+@pytest.mark.parametrize("flux", ["rusanov", "hll", "lxf"])
+def test_parity_from_two_to_the_21_cells(ora, flux):
+    """From 2^21 cells the single-phase step takes its pinned schedule (four
+    waves per SIMD).  The problem is a 4000-cell piecewise state extended to
+    2^21 + 37 cells by copies of its last cell: for fewer steps than the width
+    of its constant right end, its first 4000 cells evolve exactly as the
+    4000-cell problem does on the oracle (same dx, same dt, since the extension
+    adds no new speed), and the extension stays bit-identical."""
+    ns, steps, nx = 4000, 60, (1 << 21) + 37
+    print(f"[parity-2^21] flux={flux} nx={nx} steps={steps}", flush=True)
+    mats, S, _, ifc = _problem(ora, 1, ns, 6, seed=505)
+    assert ifc[-1] < 0.95 and (1.0 - ifc[-1]) * ns > steps + 2   # constant right end
+    Q0 = np.empty((S.shape[0], nx), order="F")
+    Q0[:, :ns] = S
+    Q0[:, ns:] = S[:, -1:]
+    Qo, to, so, dto = ora.run(1, mats, FK[flux], S, 0.6, 1.0 / nx, steps=steps)
+    Qg, r = _gpu_run(1, mats, flux, Q0, steps)
+    assert r["steps"] == so == steps
+    err = rel_err_per_component(Qg[:, :ns], Qo)
+    print(f"  max per-component rel err {err.max():.3e}", flush=True)
+    assert np.all(err <= TOL), err
+    np.testing.assert_allclose(r["dt"], dto, rtol=TOL, atol=0)
+    assert np.array_equal(Qg[:, ns:], Q0[:, ns:])
+
+
 @pytest.mark.parametrize("name,flux", [("config1_rusanov", "rusanov"), ("config1_hll", "hll")])
 def test_golden_config1_full_run(name, flux):
     """1000-cell Riemann problem run to t_end = 0.06 (701 steps) against the
