@@ -242,6 +242,77 @@ def time_adjoint(n=1 << 18, steps=2000, every=50, modes=ADJOINT_MODES):
     return out
 
 
+def time_batched_adjoint(nprob=256, lo=1024, hi=8192, steps=2000, every=50, seed=5,
+                         modes=("graph_sparse", "graph_dense")):
+    """The adjoint workload for a batch: 256 problems of floor(exp(U(ln 1024,
+    ln 8192))) cells, (rho0, c0, b0) of each scaled by U(0.8, 1.2), two seeded
+    pieces split at x = 1/2, 2000 Rusanov steps, checkpoints every 50, each
+    problem's target from a run with its b0 * 1.05 and c0 * 0.97.  The plain
+    forward is the batched solver's step loop on the same problems (one
+    captured graph); the adjoint run is the batched adjoint's checkpointed
+    forward pass plus its reverse pass (graph modes: each replayed from one
+    captured graph; eager modes: kernels launched one by one)."""
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    from hyperelasticsolver_amd.adjoint import BatchAdjoint, phase_prims_to_adjoint
+    from hyperelasticsolver_amd.batched import Batch
+    from hyperelasticsolver_amd.initial import prim2cons, random_phase_prims
+    rng = np.random.default_rng(seed)
+    ncells = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), nprob))).astype(np.int64)
+    eos, P14s = [], []
+    for _ in range(nprob):
+        sc = rng.uniform(0.8, 1.2, 3)
+        e = Barton2009(_rho0=8.93 * sc[0], _c0=4.6 * sc[1], _b0=2.1 * sc[2])
+        eos.append(e)
+        P14s.append(random_phase_prims(rng, e, 2))
+    prims = [phase_prims_to_adjoint(P) for P in P14s]
+    ifcs = [[0.5]] * nprob
+    N = int(ncells.sum())
+    eos_t = [Barton2009(_rho0=e.rho0, _b0=e.b0 * 1.05, _c0=e.c0 * 0.97) for e in eos]
+    with BatchAdjoint(eos_t, ncells, steps, every=every) as t:
+        t.set_initial(prims, ifcs)
+        t.set_target(np.zeros(N), np.zeros(N))
+        t.forward()
+        rho_t, u_t = t.profiles()
+    b = Batch(eos, ncells, flux="rusanov")
+    try:
+        sts = [prim2cons(e, P) for e, P in zip(eos, P14s)]
+        b.set_piecewise(sts, ifcs)
+        b.run(50, mode="graph")
+        b.set_piecewise(sts, ifcs)
+        fwd_ms = b.run(steps, mode="graph")
+    finally:
+        b.close()
+    runs = {}
+    grads = []
+    with BatchAdjoint(eos, ncells, steps, every=every) as a:
+        a.set_initial(prims, ifcs)
+        a.set_target(rho_t, u_t)
+        info = a.info()
+        for mode in modes:
+            a.set_launch(mode.startswith("graph"))
+            a.set_sparse(mode.endswith("sparse"))
+            a.forward()
+            a.backward()          # the first calls capture the graphs
+            losses = a.forward()
+            g = a.backward()
+            grads.append(np.concatenate([np.concatenate([x.ravel(), m]) for x, m in g]))
+            runs[mode] = {"forward_ms": a.last_forward_ms, "backward_ms": a.last_backward_ms,
+                          "ratio": (a.last_forward_ms + a.last_backward_ms) / fwd_ms}
+    if not all(np.array_equal(grads[0], x) for x in grads[1:]):
+        raise RuntimeError("batched adjoint: launch modes disagree")
+    if not np.all(np.isfinite(grads[0])):
+        raise RuntimeError("non-finite gradient")
+    out = {"problems": nprob, "cells": N, "steps": steps, "checkpoint_every": every,
+           "loss": float(losses.sum()), "forward_ms": fwd_ms, "runs": runs,
+           "device_bytes": info["device_bytes"], "checkpoints": info["checkpoints"]}
+    for kind in ("sparse", "dense"):
+        cand = [(v["ratio"], k) for k, v in runs.items() if k.endswith(kind)]
+        if cand:
+            r, k = min(cand)
+            out["fwd_plus_adj_over_fwd" + ("" if kind == "sparse" else "_dense")] = r
+    return out
+
+
 def measured_traffic(nx, flux):
     """HBM bytes per launch of the step kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE runs (calibrated on a copy kernel), when they were
@@ -267,10 +338,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="only the headline line (no HLL, batched or adjoint lines)")
-    ap.add_argument("--parts", default="hll,two_phase,transfer,batched,adjoint",
-                    help="extra lines to add: any of hll, two_phase, transfer, batched, adjoint")
+    ap.add_argument("--parts", default="hll,two_phase,transfer,batched,adjoint,adjoint_batched",
+                    help="extra lines to add: any of hll, two_phase, transfer, batched, adjoint, "
+                         "adjoint_batched")
     ap.add_argument("--adjoint-modes", default=",".join(ADJOINT_MODES))
     ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--batched-adjoint-modes", default="graph_sparse,graph_dense")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; the multi-GPU slab "
@@ -325,6 +398,9 @@ def main():
         out["batched"] = time_batched(args.batched_steps)
     if "adjoint" in parts:
         out["adjoint"] = time_adjoint(modes=tuple(m for m in args.adjoint_modes.split(",") if m))
+    if "adjoint_batched" in parts:
+        out["adjoint_batched"] = time_batched_adjoint(
+            modes=tuple(m for m in args.batched_adjoint_modes.split(",") if m))
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
         out["cpu_baseline"] = cpu_baseline(eos, Qs, ifc, 0 if args.flux == "rusanov" else 1, threads)

@@ -71,6 +71,17 @@ _SIGS = {
     "hes_adjoint_get_state": [_vp, _dp],
     "hes_adjoint_get_profiles": [_vp, _dp, _dp],
     "hes_adjoint_info": [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_ll)],
+    "hes_badjoint_create": [ctypes.POINTER(_vp), _i, ctypes.POINTER(_ll), _dp, _d, _dp, _ll, _ll],
+    "hes_badjoint_destroy": [_vp],
+    "hes_badjoint_set_initial": [_vp, ctypes.POINTER(_i), _dp, _dp],
+    "hes_badjoint_set_target": [_vp, _dp, _dp],
+    "hes_badjoint_forward": [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_float)],
+    "hes_badjoint_backward": [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_float)],
+    "hes_badjoint_set_sparse": [_vp, _i],
+    "hes_badjoint_set_launch": [_vp, _i],
+    "hes_badjoint_get_state": [_vp, _dp],
+    "hes_badjoint_get_profiles": [_vp, _dp, _dp],
+    "hes_badjoint_info": [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_ll), ctypes.POINTER(_ll)],
     "hes_batch_create": [ctypes.POINTER(_vp), _i, _i, _i, ctypes.POINTER(_ll), _dp, _dp, _dp],
     "hes_batch_destroy": [_vp],
     "hes_batch_info": [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_i), ctypes.POINTER(_i),

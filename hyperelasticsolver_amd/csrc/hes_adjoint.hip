@@ -37,153 +37,16 @@
 #include <vector>
 
 #include "hes_host.hpp"
-#include "hes_step.hpp"
-#include "hes_adjoint_gen.hpp"
+#include "hes_adjoint_common.hpp"
 #include "../../include/hes.h"
 
 using namespace hes;
+using namespace hes_adj;
 
 namespace {
 
 constexpr int ABLK = 256;  // threads per block of the tiled kernels (4 waves)
 constexpr int LBLK = 256;  // threads per block of the one-cell-per-thread kernels
-constexpr int NQ = 14;
-constexpr int NP = 13;     // differentiable primitives of a state: u(3), A(9), S
-constexpr int JL = 16;     // dlam/dQ (14), dlam/db0, dlam/dc0
-
-__device__ __forceinline__ double wave_sum(double v) {
-  // xor butterfly: every lane ends with the same bits (each level adds the
-  // same two operands on both partners)
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-// Block sum of v over ABLK threads (thread 0 gets the result).
-__device__ __forceinline__ double block_sum(double v) {
-  __shared__ double red[ABLK / 64];
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  double s = red[0];
-#pragma unroll
-  for (int k = 1; k < ABLK / 64; ++k) s += red[k];
-  return s;
-}
-
-// (value, index) with the larger value, ties to the smaller index.
-__device__ __forceinline__ void arg_better(double& v, long long& i, double ov, long long oi) {
-  if (ov > v || (ov == v && oi < i)) {
-    v = ov;
-    i = oi;
-  }
-}
-
-// Block argmax of (v, idx) over NT threads; the result is valid in thread 0.
-template <int NT>
-__device__ __forceinline__ void block_argmax(double& v, long long& idx) {
-  __shared__ double rv[NT / 64];
-  __shared__ long long ri[NT / 64];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    double ov = __shfl_xor(v, o);
-    long long oi = __shfl_xor(idx, o);
-    arg_better(v, idx, ov, oi);
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) {
-    rv[w] = v;
-    ri[w] = idx;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 1; k < NT / 64; ++k) arg_better(v, idx, rv[k], ri[k]);
-  }
-}
-
-constexpr long long NO_CELL = 0x7fffffffffffffffLL;
-
-// Max CFL speed of a state (into lmax, as double bits) and, optionally, the
-// per-block argmax (value, smallest index) of the owned cells' speeds.
-template <int NT>
-__device__ __forceinline__ void record_speeds(double lam, bool owned, long long i,
-                                              unsigned long long* lmax, double* bv,
-                                              long long* bi) {
-  double v = owned ? lam : -1.0;
-  long long idx = owned ? i : NO_CELL;
-  block_argmax<NT>(v, idx);
-  if (threadIdx.x == 0) {
-    if (lmax && v > 0.0) {
-      const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
-      // read first: most blocks find a larger maximum already there (all
-      // blocks of a step would otherwise contend for one address)
-      if (bits > __hip_atomic_load(lmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-        atomicMax(lmax, bits);
-    }
-    if (bv) {
-      bv[blockIdx.x] = v;
-      bi[blockIdx.x] = idx;
-    }
-  }
-}
-
-__device__ __forceinline__ double lmax_of(const unsigned long long* h, long long s) {
-  return __longlong_as_double((long long)h[s]);
-}
-
-// Sound speed c = sqrt(rho(Omega)) of a symmetric acoustic tensor (O11 O12 O13
-// O22 O23 O33) and dc/dO (off-diagonal entries counted twice, as each stands
-// for two matrix entries).  rho(Omega) is the eigenvalue of largest modulus,
-// found as spectral_radius_sym does; its eigenvector is the largest cross
-// product of two rows of Omega - mu I.
-__device__ __forceinline__ double speed_and_grad(const double O[6], double g[6]) {
-  const double a = O[0], b = O[3], c = O[5], d = O[1], e = O[2], f = O[4];
-  const double c2 = a + b + c;
-  const double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
-  const double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
-  const double r1 = fabs(d) + fabs(e), r2 = fabs(d) + fabs(f), r3 = fabs(e) + fabs(f);
-  const double lb = fmin(a - r1, fmin(b - r2, c - r3));
-  const double hd = 0.5 * (b - c);
-  const double lD = 0.5 * (b + c) + sqrt(hd * hd + f * f);
-  const double ww = d * d + e * e;
-  double ub = fmax(a, lD) + sqrt(ww);
-  if (a > lD) ub = fmin(ub, a + ww * rcp(a - lD));
-  double mu = cubic_top_root(c2, c1, c0, ub, lb);
-  double sgn = mu < 0.0 ? -1.0 : 1.0;
-  double rad = fabs(mu);
-  if (lb < -rad) {
-    double bot = cubic_bottom_root(c2, c1, c0, lb, ub);
-    if (fabs(bot) > rad) {
-      mu = bot;
-      rad = fabs(bot);
-      sgn = bot < 0.0 ? -1.0 : 1.0;
-    }
-  }
-  const double A = a - mu, B = b - mu, C = c - mu;
-  // rows (A d e), (d B f), (e f C)
-  double x1 = d * f - e * B, y1 = e * d - A * f, z1 = A * B - d * d;
-  double x2 = d * C - e * f, y2 = e * e - A * C, z2 = A * f - d * e;
-  double x3 = B * C - f * f, y3 = f * e - d * C, z3 = d * f - B * e;
-  double n1 = x1 * x1 + y1 * y1 + z1 * z1;
-  double n2 = x2 * x2 + y2 * y2 + z2 * z2;
-  double n3 = x3 * x3 + y3 * y3 + z3 * z3;
-  double vx = x1, vy = y1, vz = z1, nn = n1;
-  if (n2 > nn) { vx = x2; vy = y2; vz = z2; nn = n2; }
-  if (n3 > nn) { vx = x3; vy = y3; vz = z3; nn = n3; }
-  const double cs = sqrt(rad);
-  // dc/dmu = sgn / (2c), dmu/dO = v v^T / |v|^2
-  const double k = nn > 0.0 ? sgn * 0.5 * rcp(cs) * rcp(nn) : 0.0;
-  g[0] = k * vx * vx;
-  g[1] = 2.0 * k * vx * vy;
-  g[2] = 2.0 * k * vx * vz;
-  g[3] = k * vy * vy;
-  g[4] = 2.0 * k * vy * vz;
-  g[5] = k * vz * vz;
-  return cs;
-}
 
 // ---------------------------------------------------------------------------
 // Forward kernels
@@ -332,7 +195,7 @@ __global__ __launch_bounds__(LBLK) void k_adj_loss(const double* __restrict__ Q,
 #pragma unroll
     for (int r = 0; r < NQ; ++r) Qb[r * pitch + i] = ib[r];
   }
-  double s = block_sum(l);
+  double s = block_sum<LBLK>(l);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
@@ -471,46 +334,8 @@ void k_adj_step(AdjStepArgs a) {
     }
     const bool interior = inr && i > 0 && i < a.n - 1;
     const double dtdx = (a.cfl * a.dx / lmax_of(a.lmax, a.s)) / a.dx;
-    const double hdt = 0.5 * dtdx;
-    double in[hes_gen::CELL_IN];
-#pragma unroll
-    for (int r = 0; r < NQ; ++r) in[r] = q[r];
-#pragma unroll
-    for (int r = 0; r < 5; ++r) in[NQ + r] = a.gin[r];
-    double out[hes_gen::CELL_OUT];
     double ib[hes_gen::CELL_IN];
-#pragma unroll
-    for (int r = 0; r < hes_gen::CELL_IN; ++r) ib[r] = 0.0;
-    // seeds of the cell VJP from the primal (flux F, acoustic tensor, u1)
-    auto seed = [&](const double* o, double* ob) -> bool {
-      double gO[6];
-      const double c = speed_and_grad(o + NQ, gO);
-      const double u1 = o[20];
-      const double lam = fabs(u1) + c;
-      const double lam_r = lane_down(lam), lam_l = lane_up(lam);
-      const double sR = fmax(lam, lam_r), sL = fmax(lam_l, lam);
-      double sdR = 0.0, sdL = 0.0, fdot = 0.0;
-#pragma unroll
-      for (int r = 0; r < NQ; ++r) {
-        const double w = interior ? qb[r] : 0.0;
-        const double wr = lane_down(w), wl = lane_up(w);
-        const double qr = lane_down(q[r]), ql = lane_up(q[r]);
-        const double gR = wr - w, gL = w - wl;
-        const double gs = gL + gR;
-        ob[r] = hdt * gs;
-        fdot += o[r] * gs;
-        sdR += gR * (qr - q[r]);
-        sdL += gL * (q[r] - ql);
-        qb[r] += hdt * (sR * gR - sL * gL);
-      }
-      const double lbar = (lam >= lam_r ? -hdt * sdR : 0.0) + (lam > lam_l ? -hdt * sdL : 0.0);
-#pragma unroll
-      for (int k = 0; k < 6; ++k) ob[NQ + k] = lbar * gO[k];
-      ob[20] = u1 < 0.0 ? -lbar : lbar;
-      dts = owned ? 0.5 * fdot - 0.5 * sR * sdR : 0.0;
-      return __any(lbar != 0.0);
-    };
-    hes_gen::cell_fwd_vjp(in, out, seed, ib);
+    reverse_chunk(q, qb, interior, owned, 0.5 * dtdx, a.gin, ib, dts);
     bool zero = true;
     if (owned) {
 #pragma unroll
@@ -525,9 +350,9 @@ void k_adj_step(AdjStepArgs a) {
     const int z = __all(zero);
     if (lane == 0) a.zout[wave] = z;
   }
-  const double bd = block_sum(dts);
-  const double b0s = block_sum(th0);
-  const double c0s = block_sum(th1);
+  const double bd = block_sum<ABLK>(dts);
+  const double b0s = block_sum<ABLK>(th0);
+  const double c0s = block_sum<ABLK>(th1);
   if (threadIdx.x == 0) {
     a.dtp[blockIdx.x] = bd;
     a.thb[2 * blockIdx.x] += b0s;
@@ -578,7 +403,7 @@ __global__ __launch_bounds__(LBLK) void k_adj_q0sum(const double* __restrict__ Q
   for (int kk = klo; kk <= khi; ++kk) {
     for (int r = 0; r < NQ; ++r) {
       double v = (i < n && k == kk) ? Qb[r * pitch + i] : 0.0;
-      double s = block_sum(v);
+      double s = block_sum<LBLK>(v);
       if (threadIdx.x == 0) out[kk * NQ + r] = s;
     }
   }

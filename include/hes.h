@@ -181,6 +181,46 @@ int hes_adjoint_get_profiles(hes_adjoint* h, double* rho, double* u1);
 /* Device memory held by the handle and the number of stored checkpoints. */
 int hes_adjoint_info(hes_adjoint* h, long long* device_bytes, long long* checkpoints);
 
+/* ---- Adjoint of a batch of independent fixed-length single-phase Rusanov
+ *      runs (the reference has neither an adjoint nor a batch; each problem
+ *      is the forward model of hes_adjoint_*, main.jl:199-225 with
+ *      NumFluxes.jl:25-60, with its own material, dx and CFL dt).  Problem p
+ *      owns packed cells [off_p, off_p+1), off_0 = 0, off_p+1 = off_p +
+ *      ncells[p]; every problem takes the same number of steps.  Its loss is
+ *          L_p = sum_{i in p} (rho_i - rho*_i)^2 + (u1_i - u1*_i)^2
+ *      and the gradients are those of L_p with respect to its own pieces and
+ *      its own b0, c0.  Only alpha = 1, beta = 3, gamma = 2 materials. */
+typedef struct hes_badjoint hes_badjoint;
+
+/* materials: 8 Barton2009 constants per problem (8 x nproblems); dx: one per
+ * problem, or NULL for 1 / ncells[p]; nsteps: steps of every problem. */
+int hes_badjoint_create(hes_badjoint** out, int nproblems, const long long* ncells,
+                        const double* materials, double cfl, const double* dx, long long nsteps,
+                        long long every);
+int hes_badjoint_destroy(hes_badjoint* h);
+/* nstates[p] pieces per problem; prims: 13 per piece, problems in order
+ * (layout of hes_adjoint_set_initial); interfaces: nstates[p] - 1 sorted
+ * positions in (0, 1) per problem, problems in order. */
+int hes_badjoint_set_initial(hes_badjoint* h, const int* nstates, const double* prims,
+                             const double* interfaces);
+/* Target profiles rho*, u1* over the packed cells. */
+int hes_badjoint_set_target(hes_badjoint* h, const double* rho, const double* u1);
+/* Forward runs: stores the checkpoints; losses[p] = L_p (may be NULL), total = sum_p L_p. */
+int hes_badjoint_forward(hes_badjoint* h, double* losses, double* total, float* elapsed_ms);
+/* Reverse pass: grad_states (13 per piece, layout of prims), grad_material[2p],
+ * [2p + 1] = (dL_p/db0_p, dL_p/dc0_p). */
+int hes_badjoint_backward(hes_badjoint* h, double* grad_states, double* grad_material,
+                          float* elapsed_ms);
+/* As hes_adjoint_set_sparse / hes_adjoint_set_launch. */
+int hes_badjoint_set_sparse(hes_badjoint* h, int on);
+int hes_badjoint_set_launch(hes_badjoint* h, int use_graph);
+/* Final packed state (14 x total cells, Julia layout) and its rho, u1 profiles
+ * as the misfit computes them. */
+int hes_badjoint_get_state(hes_badjoint* h, double* Q_host);
+int hes_badjoint_get_profiles(hes_badjoint* h, double* rho, double* u1);
+int hes_badjoint_info(hes_badjoint* h, long long* total_cells, long long* device_bytes,
+                      long long* checkpoints);
+
 /* ---- Point-wise functions over n states (host arrays, synchronous). ---- */
 
 /* HyperelasticityMPh.cons2prim_mph (HyperelasticityMPh.jl:99-133):
