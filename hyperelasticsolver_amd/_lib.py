@@ -85,15 +85,15 @@ _SIGS = {
     "hes_batch_create": [ctypes.POINTER(_vp), _i, _i, _i, ctypes.POINTER(_ll), _dp, _dp, _dp],
     "hes_batch_destroy": [_vp],
     "hes_batch_info": [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_i), ctypes.POINTER(_i),
-                       ctypes.POINTER(_i)],
+                       ctypes.POINTER(_i), ctypes.POINTER(_i)],
     "hes_batch_set_state": [_vp, _dp, _dp],
     "hes_batch_set_piecewise": [_vp, ctypes.POINTER(_i), _dp, _dp, _dp],
     "hes_batch_get_state": [_vp, _dp],
     "hes_batch_get_times": [_vp, _dp, ctypes.POINTER(_ll)],
     "hes_batch_set_t_end": [_vp, _dp],
-    "hes_batch_set_launch": [_vp, _i, _i],
+    "hes_batch_set_launch": [_vp, _i, _i, _i],
     "hes_batch_run": [_vp, _ll, _i, ctypes.POINTER(ctypes.c_float)],
-    "hes_batch_autotune": [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i),
+    "hes_batch_autotune": [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
                            ctypes.POINTER(ctypes.c_float)],
     "hes_cons2prim": [_i, _dp, _dp, _dp, _ll],
     "hes_prim2cons": [_i, _dp, _dp, _dp, _ll],

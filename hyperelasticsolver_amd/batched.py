@@ -62,9 +62,11 @@ class Batch:
         t = ctypes.c_int()
         b = ctypes.c_int()
         x = ctypes.c_int()
+        w = ctypes.c_int()
         check(lib().hes_batch_info(self._h, ctypes.byref(n), ctypes.byref(t), ctypes.byref(b),
-                                   ctypes.byref(x)))
-        return {"cells": n.value, "tiles": t.value, "block": b.value, "xcd": x.value}
+                                   ctypes.byref(x), ctypes.byref(w)))
+        return {"cells": n.value, "tiles": t.value, "block": b.value, "xcd": x.value,
+                "tiles_per_wave": w.value}
 
     def set_state(self, Q, t0=None):
         q, n, _ = columns(Q, self.nc, "Q")
@@ -106,8 +108,10 @@ class Batch:
             np.broadcast_to(np.asarray(t_end, dtype=np.float64), (self.P,)))
         check(lib().hes_batch_set_t_end(self._h, None if v is None else ptr(v)))
 
-    def set_launch(self, block=256, xcd=True):
-        check(lib().hes_batch_set_launch(self._h, int(block), 1 if xcd else 0))
+    def set_launch(self, block=256, xcd=True, tiles_per_wave=1):
+        """Launch shape: block of 64 or 256 threads, XCD-contiguous block order,
+        and tiles of 62 cells each wave takes in turn."""
+        check(lib().hes_batch_set_launch(self._h, int(block), 1 if xcd else 0, int(tiles_per_wave)))
 
     def run(self, steps, mode="graph"):
         """Advance every problem by `steps` steps; returns the GPU time in ms."""
@@ -116,9 +120,13 @@ class Batch:
         return ms.value
 
     def autotune(self, steps=3):
+        """Time every launch shape (block, XCD order, tiles per wave) on the
+        current state, restore the state and keep the fastest."""
         b = ctypes.c_int()
         x = ctypes.c_int()
+        w = ctypes.c_int()
         ms = ctypes.c_float()
         check(lib().hes_batch_autotune(self._h, int(steps), ctypes.byref(b), ctypes.byref(x),
-                                       ctypes.byref(ms)))
-        return {"block": b.value, "xcd": x.value, "ms_per_step": ms.value}
+                                       ctypes.byref(w), ctypes.byref(ms)))
+        return {"block": b.value, "xcd": x.value, "tiles_per_wave": w.value,
+                "ms_per_step": ms.value}

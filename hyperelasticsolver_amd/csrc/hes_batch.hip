@@ -47,13 +47,17 @@ struct BatchArgs {
   int step;  // ring phase (step index modulo 6)
   int xcd;   // deal consecutive blocks to one XCD
   int lxf;   // Rusanov kernels: Lax-Friedrichs face speed dx / dt (main.jl:50)
+  // tiles per wave: the grid has nwaves = ceil(tiles / tpw) waves and wave w
+  // steps through the adjacent tiles w tpw .. w tpw + tpw - 1 (each lane takes
+  // tpw cells in turn; a tile's halo is then the tile just done)
+  int tpw, nwaves;
 };
 
+// This wave's index in the grid (wave-uniform).
 template <int BLK>
-__device__ __forceinline__ int batch_tile(const BatchArgs& a) {
+__device__ __forceinline__ int batch_wave(const BatchArgs& a) {
   long long b = a.xcd ? xcd_block() : (long long)blockIdx.x;
-  int w = (int)((b * BLK + threadIdx.x) >> 6);
-  return __builtin_amdgcn_readfirstlane(a.tile_begin + w);
+  return __builtin_amdgcn_readfirstlane((int)((b * BLK + threadIdx.x) >> 6));
 }
 
 // Max of a positive CFL speed into a problem's slot.  Fire-and-forget: a read
@@ -63,13 +67,11 @@ __device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double 
   if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
 }
 
-template <int NPH, int FLUX, int FAST, int BLK, int PIN = 0>
-__global__ __launch_bounds__(BLK)
-__attribute__((amdgpu_waves_per_eu(NPH == 1 ? 3 : 1))) void k_batch_step(BatchArgs a) {
+// One step of one tile (wave-uniform `tile`).
+template <int NPH, int FLUX, int FAST, int PIN>
+__device__ __forceinline__ void batch_tile_step(const BatchArgs& a, int tile) {
   constexpr int NC = NComp<NPH>::value;
   const int lane = threadIdx.x & 63;
-  const int tile = batch_tile<BLK>(a);
-  if (tile >= a.tile_end) return;  // the whole wave (no block-level sync here)
   const TileDesc td = a.tiles[tile];
   const int p = td.p;
   const long long lo = td.lo, hi = td.hi, first = td.first;
@@ -97,6 +99,28 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? 3 : 1))) void k_batch_step(BatchAr
       a.tring[2 * p + ((a.step + 1) & 1)] = tcur + dt;
       a.slots[3 * p + ((a.step + 2) % 3)] = 0ull;
       a.psteps[p] += 1;
+    }
+  }
+}
+
+// LOOP = 0: one tile per wave (tpw = 1); LOOP = 1: each wave steps through
+// tpw tiles.  The loop is a separate instantiation because keeping its
+// bookkeeping live costs registers (the pinned HLL step would drop from four
+// waves per SIMD to three).
+template <int NPH, int FLUX, int FAST, int BLK, int PIN = 0, int LOOP = 0>
+__global__ __launch_bounds__(BLK)
+__attribute__((amdgpu_waves_per_eu(NPH == 1 ? 3 : 1))) void k_batch_step(BatchArgs a) {
+  const int w = batch_wave<BLK>(a);
+  if constexpr (!LOOP) {
+    const int tile = a.tile_begin + w;
+    if (tile >= a.tile_end) return;  // the whole wave (no block-level sync here)
+    batch_tile_step<NPH, FLUX, FAST, PIN>(a, tile);
+  } else {
+#pragma unroll 1
+    for (int k = 0; k < a.tpw; ++k) {
+      const int tile = a.tile_begin + w * a.tpw + k;
+      if (tile >= a.tile_end) return;
+      batch_tile_step<NPH, FLUX, FAST, PIN>(a, tile);
     }
   }
 }
@@ -190,7 +214,7 @@ struct hes_batch {
   int nph, flux, nc, P, ntiles;
   long long N, pitch;
   bool fast;
-  int block, xcd;  // launch shape (autotuned)
+  int block, xcd, tpw;  // launch shape (autotuned)
   std::vector<long long> offsets_h;
   std::vector<int> tile_first_h;  // first tile of each problem (P + 1)
   double *Q[2], *C[2];
@@ -230,30 +254,38 @@ BatchArgs batch_args(hes_batch* h, long long s, int tb, int te) {
   a.step = (int)(s % 6);
   a.xcd = h->xcd;
   a.lxf = h->flux == HES_FLUX_LXF;
+  a.tpw = h->tpw;
+  a.nwaves = 0;  // set per launch
   return a;
 }
 
+template <int NPH, int FLUX, int FAST, int PIN, int LOOP>
+void launch_shape(const BatchArgs& a, int block, hipStream_t st) {
+  const unsigned nw = (unsigned)a.nwaves;
+  if (block == 64)
+    hipLaunchKernelGGL((k_batch_step<NPH, FLUX, FAST, 64, PIN, LOOP>), dim3(nw), dim3(64), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_batch_step<NPH, FLUX, FAST, 256, PIN, LOOP>), dim3((nw + 3) / 4), dim3(256),
+                       0, st, a);
+}
+
 template <int NPH, int FLUX, int FAST>
-void launch_batch_t(const BatchArgs& a, int block, hipStream_t st) {
-  int nt = a.tile_end - a.tile_begin;
-  if (nt <= 0) return;
+void launch_batch_t(const BatchArgs& a0, int block, hipStream_t st) {
+  const int tiles = a0.tile_end - a0.tile_begin;
+  if (tiles <= 0) return;
+  BatchArgs a = a0;
+  a.nwaves = (tiles + a.tpw - 1) / a.tpw;  // waves of the grid
+  const bool loop = a.tpw > 1;
   if constexpr (NPH == 1) {
     // launches over 2^21 cells and more (a whole batch; not the one-problem
     // launches): the pinned schedule of chunk_step, four waves per SIMD
-    if ((long long)nt * TILE >= (1LL << 21)) {
-      if (block == 64)
-        hipLaunchKernelGGL((k_batch_step<NPH, FLUX, FAST, 64, 1>), dim3((unsigned)nt), dim3(64), 0, st, a);
-      else
-        hipLaunchKernelGGL((k_batch_step<NPH, FLUX, FAST, 256, 1>), dim3((unsigned)((nt + 3) / 4)),
-                           dim3(256), 0, st, a);
+    if ((long long)tiles * TILE >= (1LL << 21)) {
+      loop ? launch_shape<NPH, FLUX, FAST, 1, 1>(a, block, st)
+           : launch_shape<NPH, FLUX, FAST, 1, 0>(a, block, st);
       return;
     }
   }
-  if (block == 64)
-    hipLaunchKernelGGL((k_batch_step<NPH, FLUX, FAST, 64>), dim3((unsigned)nt), dim3(64), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_batch_step<NPH, FLUX, FAST, 256>), dim3((unsigned)((nt + 3) / 4)),
-                       dim3(256), 0, st, a);
+  loop ? launch_shape<NPH, FLUX, FAST, 0, 1>(a, block, st) : launch_shape<NPH, FLUX, FAST, 0, 0>(a, block, st);
 }
 
 void launch_batch(hes_batch* h, const BatchArgs& a) {
@@ -339,6 +371,7 @@ int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const
   h->P = nproblems;
   h->block = 256;
   h->xcd = 1;
+  h->tpw = 1;
   h->offsets_h.resize(nproblems + 1);
   h->tile_first_h.resize(nproblems + 1);
   h->offsets_h[0] = 0;
@@ -403,12 +436,14 @@ int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const
   return HES_OK;
 }
 
-int hes_batch_info(hes_batch* h, long long* total_cells, int* ntiles, int* block, int* xcd) {
+int hes_batch_info(hes_batch* h, long long* total_cells, int* ntiles, int* block, int* xcd,
+                   int* tiles_per_wave) {
   if (!h) return fail(HES_ERR_ARG, "null handle");
   if (total_cells) *total_cells = h->N;
   if (ntiles) *ntiles = h->ntiles;
   if (block) *block = h->block;
   if (xcd) *xcd = h->xcd;
+  if (tiles_per_wave) *tiles_per_wave = h->tpw;
   return HES_OK;
 }
 
@@ -513,11 +548,13 @@ int hes_batch_get_times(hes_batch* h, double* t, long long* steps) {
   return HES_OK;
 }
 
-int hes_batch_set_launch(hes_batch* h, int block, int xcd) {
+int hes_batch_set_launch(hes_batch* h, int block, int xcd, int tiles_per_wave) {
   if (!h) return fail(HES_ERR_ARG, "null handle");
   if (block != 64 && block != 256) return fail(HES_ERR_ARG, "block must be 64 or 256");
+  if (tiles_per_wave < 1 || tiles_per_wave > 64) return fail(HES_ERR_ARG, "tiles per wave must be 1..64");
   h->block = block;
   h->xcd = xcd ? 1 : 0;
+  h->tpw = tiles_per_wave;
   return HES_OK;
 }
 
@@ -576,7 +613,8 @@ int hes_batch_run(hes_batch* h, long long steps, int mode, float* elapsed_ms) {
 
 // Try every launch shape for `steps` steps on a copy of the current state and
 // keep the fastest.  The state, times and CFL rings are restored afterwards.
-int hes_batch_autotune(hes_batch* h, int steps, int* best_block, int* best_xcd, float* best_ms) {
+int hes_batch_autotune(hes_batch* h, int steps, int* best_block, int* best_xcd, int* best_tpw,
+                       float* best_ms) {
   if (!h || steps < 1) return fail(HES_ERR_ARG, "bad argument");
   if (!h->has_state) return fail(HES_ERR_ARG, "no state set");
   long long s0 = h->step;
@@ -596,25 +634,27 @@ int hes_batch_autotune(hes_batch* h, int steps, int* best_block, int* best_xcd, 
   HIP_TRY(copy_on(h->stream, sl0, h->slots, sizeof(unsigned long long) * 3 * h->P, dd));
   HIP_TRY(copy_on(h->stream, ps0, h->psteps, sizeof(int) * h->P, dd));
   const int blocks[2] = {64, 256};
+  const int tpws[3] = {1, 2, 4};
   float best = 1e30f;
-  int bb = h->block, bx = h->xcd, rc = HES_OK;
-  for (int bi = 0; bi < 2 && rc == HES_OK; ++bi)
-    for (int x = 0; x < 2 && rc == HES_OK; ++x) {
-      h->block = blocks[bi];
-      h->xcd = x;
-      float ms = 0.f;
-      rc = hes_batch_run(h, 1, 0, &ms);  // warm-up
-      if (rc == HES_OK) rc = hes_batch_run(h, steps, 0, &ms);
-      if (rc == HES_OK && ms < best) { best = ms; bb = blocks[bi]; bx = x; }
-      // restore the starting state for the next candidate
-      h->step = s0;
-      if (copy_on(h->stream, h->Q[s0 & 1], q0, qb, dd) != hipSuccess ||
-          copy_on(h->stream, h->C[s0 & 1], c0, cb, dd) != hipSuccess ||
-          copy_on(h->stream, h->tring, t0, sizeof(double) * 2 * h->P, dd) != hipSuccess ||
-          copy_on(h->stream, h->slots, sl0, sizeof(unsigned long long) * 3 * h->P, dd) != hipSuccess ||
-          copy_on(h->stream, h->psteps, ps0, sizeof(int) * h->P, dd) != hipSuccess)
-        rc = fail(HES_ERR_HIP, "autotune restore failed");
-    }
+  int bb = h->block, bx = h->xcd, bt = h->tpw, rc = HES_OK;
+  for (int c = 0; c < 12 && rc == HES_OK; ++c) {
+    const int bi = c / 6, x = (c / 3) % 2, ti = c % 3;
+    h->block = blocks[bi];
+    h->xcd = x;
+    h->tpw = tpws[ti];
+    float ms = 0.f;
+    rc = hes_batch_run(h, 1, 0, &ms);  // warm-up
+    if (rc == HES_OK) rc = hes_batch_run(h, steps, 0, &ms);
+    if (rc == HES_OK && ms < best) { best = ms; bb = blocks[bi]; bx = x; bt = tpws[ti]; }
+    // restore the starting state for the next candidate
+    h->step = s0;
+    if (copy_on(h->stream, h->Q[s0 & 1], q0, qb, dd) != hipSuccess ||
+        copy_on(h->stream, h->C[s0 & 1], c0, cb, dd) != hipSuccess ||
+        copy_on(h->stream, h->tring, t0, sizeof(double) * 2 * h->P, dd) != hipSuccess ||
+        copy_on(h->stream, h->slots, sl0, sizeof(unsigned long long) * 3 * h->P, dd) != hipSuccess ||
+        copy_on(h->stream, h->psteps, ps0, sizeof(int) * h->P, dd) != hipSuccess)
+      rc = fail(HES_ERR_HIP, "autotune restore failed");
+  }
   (void)hipFree(q0);
   (void)hipFree(c0);
   (void)hipFree(t0);
@@ -622,8 +662,10 @@ int hes_batch_autotune(hes_batch* h, int steps, int* best_block, int* best_xcd, 
   (void)hipFree(ps0);
   h->block = bb;
   h->xcd = bx;
+  h->tpw = bt;
   if (best_block) *best_block = bb;
   if (best_xcd) *best_xcd = bx;
+  if (best_tpw) *best_tpw = bt;
   if (best_ms) *best_ms = best / (float)steps;
   return rc;
 }

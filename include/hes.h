@@ -113,7 +113,8 @@ typedef struct hes_batch hes_batch;
 int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const long long* ncells,
                      const double* materials, const double* cfl, const double* dx);
 int hes_batch_destroy(hes_batch* h);
-int hes_batch_info(hes_batch* h, long long* total_cells, int* ntiles, int* block, int* xcd);
+int hes_batch_info(hes_batch* h, long long* total_cells, int* ntiles, int* block, int* xcd,
+                   int* tiles_per_wave);
 /* t0: per problem start times, or NULL for 0. */
 int hes_batch_set_state(hes_batch* h, const double* Q_host, const double* t0);
 /* Per-problem piecewise-constant state built on the device: problem p has
@@ -127,15 +128,19 @@ int hes_batch_get_times(hes_batch* h, double* t, long long* steps);
 /* Per-problem end time (main.jl:202 `while t < T`), NULL for none. */
 int hes_batch_set_t_end(hes_batch* h, const double* t_end);
 /* Launch shape: block of 64 or 256 threads; xcd != 0 deals consecutive blocks
- * to one XCD. */
-int hes_batch_set_launch(hes_batch* h, int block, int xcd);
+ * to one XCD; tiles_per_wave (1..64) adjacent tiles of 62 cells per wave in
+ * turn (a grid of ceil(tiles / tiles_per_wave) waves, each lane taking that
+ * many cells). */
+int hes_batch_set_launch(hes_batch* h, int block, int xcd, int tiles_per_wave);
 /* Advance every problem by `steps` steps.  mode 0: one launch per step;
  * 1: the whole step loop captured in one hipGraph and launched once;
  * 2: one launch per problem per step (the per-problem loop). */
 int hes_batch_run(hes_batch* h, long long steps, int mode, float* elapsed_ms);
-/* Time every launch shape for `steps` steps on the current state (which is
- * restored afterwards) and keep the fastest. */
-int hes_batch_autotune(hes_batch* h, int steps, int* block, int* xcd, float* ms_per_step);
+/* Time every launch shape (block 64/256 x XCD order on/off x 1, 2 or 4 tiles
+ * per wave) for `steps` steps on the current state (which is restored
+ * afterwards) and keep the fastest. */
+int hes_batch_autotune(hes_batch* h, int steps, int* block, int* xcd, int* tiles_per_wave,
+                       float* ms_per_step);
 
 /* ---- Adjoint of a fixed-length single-phase Rusanov run (the reference has
  *      no adjoint; its forward model is main.jl:199-225 with NumFluxes.jl:25-60

@@ -96,15 +96,17 @@ def test_batch_over_two_to_the_21_cells(ora, flux, fk):
     steps = 8
     print(f"[batch] over 2^21 cells ({sum(ncells)}), flux={flux}", flush=True)
     res = []
-    for mode in ("graph", "loop"):
+    for mode, tpw in (("graph", 1), ("loop", 1), ("graph", 3)):
         b, eos, sts, ifs = _build(ora, ncells, flux=flux, seed=4)
+        b.set_launch(256, True, tpw)
         Q0 = b.get_state()
         b.run(steps, mode=mode)
         res.append((b.get_state(), b.times()))
         offs = list(b.offsets)
         b.close()
-    assert np.array_equal(res[0][0], res[1][0])
-    assert np.array_equal(res[0][1][0], res[1][1][0])
+    for r in res[1:]:
+        assert np.array_equal(res[0][0], r[0])
+        assert np.array_equal(res[0][1][0], r[1][0])
     for p in (0, 17, 47):
         n = ncells[p]
         Qo, _, so, _ = ora.run(1, materials_array(eos[p]), fk, Q0[:, offs[p]:offs[p + 1]], 0.6,
@@ -156,7 +158,7 @@ def test_batch_modes_agree_and_autotune_restores(ora):
         b, *_ = _build(ora, ncells, seed=4)
         if mode == "graph":
             tuned = b.autotune(steps=2)
-            assert tuned["block"] in (64, 256)
+            assert tuned["block"] in (64, 256) and tuned["tiles_per_wave"] in (1, 2, 4)
         b.run(9, mode=mode)
         res.append((b.get_state(), b.times()))
         b.close()
@@ -164,6 +166,30 @@ def test_batch_modes_agree_and_autotune_restores(ora):
         assert np.array_equal(Q, res[0][0])
         assert np.array_equal(t, res[0][1][0])
         assert np.array_equal(s, res[0][1][1])
+
+
+def test_every_launch_shape_gives_the_same_bits(ora):
+    """Block size, XCD block order and tiles per wave (each wave stepping
+    through its tiles in turn) change only the schedule: every shape gives the
+    bits of the default one, eager and graph-replayed."""
+    print("[batch] launch shapes", flush=True)
+    ncells = [70, 5, 300, 1000, 62, 2500, 63, 64]
+    ref = None
+    for block, xcd, tpw, mode in ((256, True, 1, "eager"), (64, False, 1, "graph"),
+                                  (64, True, 2, "eager"), (256, False, 3, "graph"),
+                                  (256, True, 4, "graph"), (64, True, 7, "eager"),
+                                  (64, False, 64, "graph")):
+        b, *_ = _build(ora, ncells, seed=8)
+        b.set_launch(block, xcd, tpw)
+        assert b.info()["tiles_per_wave"] == tpw
+        b.run(11, mode=mode)
+        out = (b.get_state(), b.times())
+        b.close()
+        if ref is None:
+            ref = out
+        else:
+            assert np.array_equal(out[0], ref[0]), (block, xcd, tpw, mode)
+            assert np.array_equal(out[1][0], ref[1][0]) and np.array_equal(out[1][1], ref[1][1])
 
 
 def test_batch_per_problem_t_end(ora):
