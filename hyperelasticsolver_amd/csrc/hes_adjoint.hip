@@ -845,6 +845,10 @@ int hes_adjoint_set_initial(hes_adjoint* h, int nstates, const double* prims,
     bounds[k] = lo;
   }
   if (nstates != h->nst) {
+    // the old piece buffers go: until the new ones are in place the handle
+    // has no initial state (a failed allocation leaves it so)
+    h->has_init = false;
+    h->fwd_done = false;
     (void)hipStreamSynchronize(h->stream);
     if (h->gf) { (void)hipGraphExecDestroy(h->gf); h->gf = nullptr; }
     if (h->gb) { (void)hipGraphExecDestroy(h->gb); h->gb = nullptr; }

@@ -932,6 +932,10 @@ int hes_badjoint_set_initial(hes_badjoint* h, const int* nstates, const double* 
   bounds[npieces] = h->N;
   (void)hipStreamSynchronize(h->stream);
   drop_graphs(h);
+  // until the new pieces are in place the handle has no initial state (a
+  // failed allocation or copy leaves it so)
+  h->has_init = false;
+  h->fwd_done = false;
   if (npieces != h->npieces) {
     void* old[] = {h->poff, h->piece_prob, h->bounds, h->prims, h->states, h->sums, h->gstates};
     for (void* q : old)
