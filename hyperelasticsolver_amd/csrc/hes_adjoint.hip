@@ -728,7 +728,7 @@ int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* materi
   if (nsteps > 0x7fffffffLL) return fail(HES_ERR_ARG, "too many steps");
   hes_adjoint* h = new hes_adjoint();
   h->n = ncells;
-  h->pitch = (ncells + 63) / 64 * 64;
+  h->pitch = (ncells + 63) / 64 * 64 + 256;  // rows padded off a power-of-two stride, as the solver's
   h->nsteps = nsteps;
   h->every = every < nsteps ? every : nsteps;
   h->nck = (nsteps + h->every - 1) / h->every;

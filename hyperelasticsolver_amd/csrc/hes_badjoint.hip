@@ -806,7 +806,7 @@ int hes_badjoint_create(hes_badjoint** out, int nproblems, const long long* ncel
   h->tfirst_h[nproblems] = (int)tdesc.size();
   h->ntiles = (int)tdesc.size();
   h->N = h->offs_h[nproblems];
-  h->pitch = (h->N + 63) / 64 * 64;
+  h->pitch = (h->N + 63) / 64 * 64 + 256;  // rows padded off a power-of-two stride, as the solver's
   const size_t Pt = (size_t)h->pitch;
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);

@@ -51,6 +51,17 @@ struct BatchArgs {
   // steps through the adjacent tiles w tpw .. w tpw + tpw - 1 (each lane takes
   // tpw cells in turn; a tile's halo is then the tile just done)
   int tpw, nwaves;
+  // two phases: the half path integrals of the launch's jump faces
+  // (k_bpath2, k_bpath2_hll; face i+1/2 at [r * pitch + i]) and the jump-face
+  // list they come from (k_bmark2): face index and problem per entry, counted
+  // in jcnt[jslot].  The slot alternates per launch; each scan clears the
+  // other one for the next launch.
+  const double* dpath;
+  unsigned long long* jcnt;
+  long long* jlist;
+  int* jprob;
+  long long jcap;
+  int jslot;
 };
 
 // This wave's index in the grid (wave-uniform).
@@ -80,7 +91,7 @@ __device__ __forceinline__ void batch_tile_step(const BatchArgs& a, int tile) {
   // copy whose address is passed on would live in scratch memory
   const MatDev* m = a.mats + NPH * p;
   const ChunkCtx ctx = {m,  a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, lo, hi, a.cfl[p], a.dx[p],
-                        0.0, nullptr, a.lxf};
+                        0.0, NPH == 2 ? a.dpath : nullptr, a.lxf};
   const long long i = first + lane - 1;
   double q[NC], cc[NPH];
   load_chunk<NPH>(ctx, i, q, cc);  // in flight while the problem's time is read
@@ -90,7 +101,8 @@ __device__ __forceinline__ void batch_tile_step(const BatchArgs& a, int tile) {
     return;
   }
   const double lmax = __longlong_as_double(*(const long long*)(a.slots + 3 * p + (a.step % 3)));
-  double lam = chunk_step<NPH, FLUX, FAST, false, PIN != 0>(ctx, i, q, cc, lmax);
+  // two phases: the path terms of the jump faces come from k_bpath2 / k_bpath2_hll
+  double lam = chunk_step<NPH, FLUX, FAST, NPH == 2, PIN != 0>(ctx, i, q, cc, lmax);
   lam = wave_max(lam);
   if (lane == 0) {
     atomic_max_pos(a.slots + 3 * p + ((a.step + 1) % 3), lam);
@@ -109,7 +121,7 @@ __device__ __forceinline__ void batch_tile_step(const BatchArgs& a, int tile) {
 // waves per SIMD to three).
 template <int NPH, int FLUX, int FAST, int BLK, int PIN = 0, int LOOP = 0>
 __global__ __launch_bounds__(BLK)
-__attribute__((amdgpu_waves_per_eu(NPH == 1 ? 3 : 1))) void k_batch_step(BatchArgs a) {
+__attribute__((amdgpu_waves_per_eu(3))) void k_batch_step(BatchArgs a) {
   const int w = batch_wave<BLK>(a);
   if constexpr (!LOOP) {
     const int tile = a.tile_begin + w;
@@ -123,6 +135,59 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? 3 : 1))) void k_batch_step(BatchAr
       batch_tile_step<NPH, FLUX, FAST, PIN>(a, tile);
     }
   }
+}
+
+// Two phases: the faces with a volume-fraction jump in the launch's tiles
+// (face i+1/2 for each owned cell i < hi - 1 of a problem that steps),
+// appended with their problems to jlist / jprob and counted in jcnt[jslot];
+// block 0 clears the other slot for the next launch.  One wave per tile, four
+// per block, a ballot and one atomic per wave; the state and the problem's
+// time are loaded together.
+__global__ __launch_bounds__(256) void k_bmark2(BatchArgs a) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.jcnt[a.jslot ^ 1] = 0ull;
+  const int tile =
+      __builtin_amdgcn_readfirstlane(a.tile_begin + (int)((blockIdx.x * 256 + threadIdx.x) >> 6));
+  if (tile >= a.tile_end) return;  // the whole wave
+  const TileDesc td = a.tiles[tile];
+  const int p = td.p;
+  const int lane = threadIdx.x & 63;
+  const long long i = td.first + lane;
+  const long long ic = i < td.hi - 1 ? i : td.hi - 2;  // a face of the problem on every lane
+  const bool jump_here = face_jump2(a.Qin, ic, a.pitch);
+  const bool live = a.tring[2 * p + (a.step & 1)] < a.t_end[p];  // the problem steps
+  const bool jump = live && lane < TILE && i < td.hi - 1 && jump_here;
+  const unsigned long long m = __ballot(jump);
+  if (m == 0ull) return;
+  unsigned long long base = 0ull;
+  if (lane == 0) base = atomicAdd(a.jcnt + a.jslot, (unsigned long long)__popcll(m));
+  base = __shfl(base, 0);
+  const unsigned long long pos = base + __popcll(m & ((1ull << lane) - 1ull));
+  if (jump && pos < (unsigned long long)a.jcap) {
+    a.jlist[pos] = i;
+    a.jprob[pos] = p;
+  }
+}
+
+// The path terms of the listed jump faces with each face's own materials, 16
+// lanes per face (as the one-problem k_path2 / k_path2_hll).
+template <int FAST>
+__global__ __launch_bounds__(64) void k_bpath2(BatchArgs a) {
+  const unsigned long long cnt = min(a.jcnt[a.jslot], (unsigned long long)a.jcap);
+  const unsigned long long stride = (unsigned long long)gridDim.x * 4;
+  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 4; j < cnt;
+       j += stride)
+    face_path2<FAST>(a.mats + 2 * a.jprob[j], a.Qin, (double*)a.dpath, a.jlist[j], a.pitch,
+                     threadIdx.x & 15);
+}
+
+template <int FAST>
+__global__ __launch_bounds__(64) void k_bpath2_hll(BatchArgs a) {
+  const unsigned long long cnt = min(a.jcnt[a.jslot], (unsigned long long)a.jcap);
+  const unsigned long long stride = (unsigned long long)gridDim.x * 4;
+  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 4; j < cnt;
+       j += stride)
+    face_path2_hll<FAST>(a.mats + 2 * a.jprob[j], a.Qin, a.Cin, (double*)a.dpath, a.jlist[j],
+                         a.pitch, threadIdx.x & 15);
 }
 
 // Sound speeds and per-problem CFL maximum of the current state.
@@ -224,6 +289,12 @@ struct hes_batch {
   double *cfl, *dx, *t_end, *tring;
   unsigned long long* slots;
   int *psteps, *status;
+  // two phases: path-term buffer and jump-face list (see BatchArgs)
+  double* dpath;
+  unsigned long long* jcnt;
+  long long* jlist;
+  int* jprob;
+  int jpar;  // count slot of the next launch
   long long step;
   hipStream_t stream;
   hipEvent_t ev0, ev1;
@@ -256,6 +327,12 @@ BatchArgs batch_args(hes_batch* h, long long s, int tb, int te) {
   a.lxf = h->flux == HES_FLUX_LXF;
   a.tpw = h->tpw;
   a.nwaves = 0;  // set per launch
+  a.dpath = h->dpath;
+  a.jcnt = h->jcnt;
+  a.jlist = h->jlist;
+  a.jprob = h->jprob;
+  a.jcap = h->N;
+  a.jslot = h->jpar;
   return a;
 }
 
@@ -298,12 +375,25 @@ void launch_batch(hes_batch* h, const BatchArgs& a) {
       h->fast ? launch_batch_t<1, FLUX_HLL, 1>(a, h->block, h->stream)
               : launch_batch_t<1, FLUX_HLL, 0>(a, h->block, h->stream);
   } else {
-    if (h->flux != HES_FLUX_HLL)
-      h->fast ? launch_batch_t<2, FLUX_RUSANOV, 1>(a, h->block, h->stream)
-              : launch_batch_t<2, FLUX_RUSANOV, 0>(a, h->block, h->stream);
-    else
-      h->fast ? launch_batch_t<2, FLUX_HLL, 1>(a, h->block, h->stream)
-              : launch_batch_t<2, FLUX_HLL, 0>(a, h->block, h->stream);
+    const int tiles = a.tile_end - a.tile_begin;
+    if (tiles <= 0) return;
+    // jump faces first (a light scan), then their path terms over a fixed
+    // grid of at most 1024 waves, then the step
+    hipStream_t st = h->stream;
+    hipLaunchKernelGGL(k_bmark2, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, st, a);
+    const unsigned nbl = (unsigned)std::min<long long>(((long long)tiles * TILE + 254) / 256, 1024);
+    if (h->flux != HES_FLUX_HLL) {
+      if (h->fast) hipLaunchKernelGGL(k_bpath2<1>, dim3(nbl), dim3(64), 0, st, a);
+      else hipLaunchKernelGGL(k_bpath2<0>, dim3(nbl), dim3(64), 0, st, a);
+      h->fast ? launch_batch_t<2, FLUX_RUSANOV, 1>(a, h->block, st)
+              : launch_batch_t<2, FLUX_RUSANOV, 0>(a, h->block, st);
+    } else {
+      if (h->fast) hipLaunchKernelGGL(k_bpath2_hll<1>, dim3(nbl), dim3(64), 0, st, a);
+      else hipLaunchKernelGGL(k_bpath2_hll<0>, dim3(nbl), dim3(64), 0, st, a);
+      h->fast ? launch_batch_t<2, FLUX_HLL, 1>(a, h->block, st)
+              : launch_batch_t<2, FLUX_HLL, 0>(a, h->block, st);
+    }
+    h->jpar ^= 1;  // this launch's scan cleared the other slot
   }
 }
 
@@ -333,6 +423,8 @@ int batch_after_set(hes_batch* h, const double* t0) {
   HIP_TRY(copy_on(h->stream, h->tring, tr.data(), sizeof(double) * 2 * h->P, hipMemcpyHostToDevice));
   HIP_TRY(zero_on(h->stream, h->psteps, sizeof(int) * h->P));
   HIP_TRY(zero_on(h->stream, h->status, sizeof(int) * 4));
+  if (h->jcnt) HIP_TRY(zero_on(h->stream, h->jcnt, 2 * sizeof(unsigned long long)));
+  h->jpar = 0;
   h->step = 0;
   h->has_state = true;
   return batch_init_speeds(h);
@@ -345,8 +437,9 @@ extern "C" {
 int hes_batch_destroy(hes_batch* h) {
   if (!h) return HES_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  void* ptrs[] = {h->Q[0], h->Q[1], h->C[0], h->C[1], h->tiles, h->offsets,
-                  h->mats, h->cfl, h->dx, h->t_end, h->tring, h->slots, h->psteps, h->status};
+  void* ptrs[] = {h->Q[0], h->Q[1], h->C[0], h->C[1], h->tiles, h->offsets, h->mats, h->cfl,
+                  h->dx, h->t_end, h->tring, h->slots, h->psteps, h->status, h->dpath, h->jcnt,
+                  h->jlist, h->jprob};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -401,7 +494,7 @@ int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const
   h->tile_first_h[nproblems] = (int)tstart.size();
   h->ntiles = (int)tstart.size();
   h->N = h->offsets_h[nproblems];
-  h->pitch = (h->N + 63) / 64 * 64;
+  h->pitch = (h->N + 63) / 64 * 64 + 256;  // rows padded off a power-of-two stride, as the solver's
   size_t qb = sizeof(double) * (size_t)h->nc * h->pitch, cb = sizeof(double) * (size_t)nphase * h->pitch;
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   for (int b = 0; b < 2 && e == hipSuccess; ++b) {
@@ -423,6 +516,13 @@ int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const
   up((void**)&h->t_end, vte.data(), sizeof(double) * nproblems);
   up((void**)&h->tring, nullptr, sizeof(double) * 2 * nproblems);
   up((void**)&h->slots, nullptr, sizeof(unsigned long long) * 3 * nproblems);
+  if (nphase == 2) {
+    up((void**)&h->dpath, nullptr, 2 * qb);  // D-, D+ rows
+    up((void**)&h->jcnt, nullptr, 2 * sizeof(unsigned long long));
+    up((void**)&h->jlist, nullptr, sizeof(long long) * (size_t)h->N);
+    up((void**)&h->jprob, nullptr, sizeof(int) * (size_t)h->N);
+    if (e == hipSuccess) e = zero_on(h->stream, h->jcnt, 2 * sizeof(unsigned long long));
+  }
   up((void**)&h->psteps, nullptr, sizeof(int) * nproblems);
   up((void**)&h->status, nullptr, sizeof(int) * 4);
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);

@@ -150,6 +150,48 @@ def test_batch_two_phase_matches_oracle(ora, flux, fk):
         assert np.all(err <= TOL), (p, err.max())
 
 
+@pytest.mark.parametrize("flux", ["rusanov", "hll", "lxf"])
+def test_two_phase_batch_equals_single_problem_runs(ora, flux):
+    """Two-phase problems take the jump-face scan and path-term prepass per
+    launch, with each face's own materials.  Every launch mode and shape gives
+    the same bits; every problem takes the one-problem solver's steps and
+    times exactly, also when the problems stop at different end times, and its
+    state equals the solver's: bit for bit for Rusanov and Lax-Friedrichs, to
+    rounding (1e-13 relative) for HLL, whose path code the two kernels may
+    contract into fused multiply-adds differently."""
+    from hyperelasticsolver_amd.solver import Solver
+    print(f"[batch] two-phase vs single runs, flux={flux}", flush=True)
+    ncells = [3, 61, 62, 63, 64, 130, 300]
+    steps = 9
+    t_end = [np.inf, np.inf, 2e-4, np.inf, 5e-4, np.inf, 3e-4]
+    res = []
+    for mode, shape in (("graph", (256, True, 1)), ("eager", (64, False, 3)), ("loop", (64, True, 1))):
+        b, eos, sts, ifs = _build(ora, ncells, nph=2, flux=flux, seed=9)
+        b.set_launch(*shape)
+        b.set_t_end(t_end)
+        Q0 = b.get_state()
+        b.run(steps, mode=mode)
+        res.append((b.get_state(), b.times()))
+        offs = list(b.offsets)
+        b.close()
+    for r in res[1:]:
+        assert np.array_equal(r[0], res[0][0]) and np.array_equal(r[1][1], res[0][1][1])
+    Qb, (tb, sb) = res[0]
+    assert any(s < steps for s in sb) and any(s == steps for s in sb)
+    for p, n in enumerate(ncells):
+        with Solver(eos[p], n, flux=flux, cfl=0.6) as s:
+            s.set_state(Q0[:, offs[p]:offs[p + 1]])
+            r = s.run(steps=steps, t_end=t_end[p], use_graph=False)
+            Qs = s.get_state()
+        assert sb[p] == r["steps"] and tb[p] == r["t"], (p, sb[p], r["steps"])
+        err = rel_err_per_component(Qb[:, offs[p]:offs[p + 1]], Qs)
+        print(f"  problem {p}: max rel diff to the solver {err.max():.2e}", flush=True)
+        if flux == "hll":
+            assert np.all(err <= 1e-13), (p, err.max())
+        else:
+            assert np.array_equal(Qb[:, offs[p]:offs[p + 1]], Qs), f"problem {p}"
+
+
 def test_batch_modes_agree_and_autotune_restores(ora):
     print("[batch] modes + autotune", flush=True)
     ncells = [70, 5, 300, 1000, 62]
