@@ -171,6 +171,28 @@ def time_batched(steps, loop_steps=5):
             "autotune": tuned}
 
 
+def time_batched_two_phase(nprob=512, n=4096, steps=20, loop_steps=3):
+    """512 two-phase problems of 4096 cells (2^21 cells in all), each the
+    two-phase workload's four seeded pieces, HLL and Rusanov; one launch
+    sequence (jump-face scan, path terms, step) per step over the whole batch,
+    replayed from a graph, against one sequence per problem per step."""
+    from hyperelasticsolver_amd.batched import Batch
+    eos, Qs, ifc = two_phase_problem(1 << 21)
+    out = {"problems": nprob, "cells": nprob * n, "bytes_per_cell_update": BYTES_PER_CELL_2PH}
+    for flux in ("hll", "rusanov"):
+        with Batch([eos] * nprob, [n] * nprob, flux=flux) as b:
+            b.set_piecewise([Qs] * nprob, [ifc] * nprob)
+            b.run(2, mode="graph")
+            ms = b.run(steps, mode="graph") / steps
+            loop_ms = b.run(loop_steps, mode="loop") / loop_steps
+            if not np.all(np.isfinite(b.get_state())):
+                raise RuntimeError("non-finite batched two-phase state")
+        out[flux] = {"ms_per_step": ms, "value": nprob * n / (ms / 1e3),
+                     "hbm_frac": BYTES_PER_CELL_2PH * nprob * n / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                     "loop_ms_per_step": loop_ms, "speedup_vs_one_problem_per_launch": loop_ms / ms}
+    return out
+
+
 def adjoint_problem(n, steps, every, seed=2):
     """Two seeded pieces split at x = 1/2 (A = I + 0.05 N(0,1), u ~ U(-1/2,1/2)^3,
     S = 1e-3) and the target profiles of a run with b0 * 1.05 and c0 * 0.97."""
@@ -338,9 +360,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="only the headline line (no HLL, batched or adjoint lines)")
-    ap.add_argument("--parts", default="hll,two_phase,transfer,batched,adjoint,adjoint_batched",
-                    help="extra lines to add: any of hll, two_phase, transfer, batched, adjoint, "
-                         "adjoint_batched")
+    ap.add_argument("--parts",
+                    default="hll,two_phase,transfer,batched,batched_two_phase,adjoint,adjoint_batched",
+                    help="extra lines to add: any of hll, two_phase, transfer, batched, "
+                         "batched_two_phase, adjoint, adjoint_batched")
     ap.add_argument("--adjoint-modes", default=",".join(ADJOINT_MODES))
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--batched-adjoint-modes", default="graph_sparse,graph_dense")
@@ -396,6 +419,8 @@ def main():
         out["host_transfer"] = host_transfer(eos, Qs, ifc, nx)
     if "batched" in parts:
         out["batched"] = time_batched(args.batched_steps)
+    if "batched_two_phase" in parts:
+        out["batched_two_phase"] = time_batched_two_phase()
     if "adjoint" in parts:
         out["adjoint"] = time_adjoint(modes=tuple(m for m in args.adjoint_modes.split(",") if m))
     if "adjoint_batched" in parts:

@@ -140,28 +140,28 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_batch_step(BatchArgs a) {
 // Two phases: the faces with a volume-fraction jump in the launch's tiles
 // (face i+1/2 for each owned cell i < hi - 1 of a problem that steps),
 // appended with their problems to jlist / jprob and counted in jcnt[jslot];
-// block 0 clears the other slot for the next launch.  One wave per tile, four
-// per block, a ballot and one atomic per wave; the state and the problem's
-// time are loaded together.
-__global__ __launch_bounds__(256) void k_bmark2(BatchArgs a) {
+// block 0 clears the other slot for the next launch.  One wave per tile, 16
+// per block, one atomic per block; the state and the problem's time are
+// loaded together.
+constexpr int MARK_BLK = 1024;
+__global__ __launch_bounds__(MARK_BLK) void k_bmark2(BatchArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0) a.jcnt[a.jslot ^ 1] = 0ull;
-  const int tile =
-      __builtin_amdgcn_readfirstlane(a.tile_begin + (int)((blockIdx.x * 256 + threadIdx.x) >> 6));
-  if (tile >= a.tile_end) return;  // the whole wave
-  const TileDesc td = a.tiles[tile];
-  const int p = td.p;
+  const int tile = __builtin_amdgcn_readfirstlane(
+      a.tile_begin + (int)((blockIdx.x * MARK_BLK + threadIdx.x) >> 6));
   const int lane = threadIdx.x & 63;
-  const long long i = td.first + lane;
-  const long long ic = i < td.hi - 1 ? i : td.hi - 2;  // a face of the problem on every lane
-  const bool jump_here = face_jump2(a.Qin, ic, a.pitch);
-  const bool live = a.tring[2 * p + (a.step & 1)] < a.t_end[p];  // the problem steps
-  const bool jump = live && lane < TILE && i < td.hi - 1 && jump_here;
-  const unsigned long long m = __ballot(jump);
-  if (m == 0ull) return;
-  unsigned long long base = 0ull;
-  if (lane == 0) base = atomicAdd(a.jcnt + a.jslot, (unsigned long long)__popcll(m));
-  base = __shfl(base, 0);
-  const unsigned long long pos = base + __popcll(m & ((1ull << lane) - 1ull));
+  bool jump = false;
+  long long i = 0;
+  int p = 0;
+  if (tile < a.tile_end) {
+    const TileDesc td = a.tiles[tile];
+    p = td.p;
+    i = td.first + lane;
+    const long long ic = i < td.hi - 1 ? i : td.hi - 2;  // a face of the problem on every lane
+    const bool jump_here = face_jump2(a.Qin, ic, a.pitch);
+    const bool live = a.tring[2 * p + (a.step & 1)] < a.t_end[p];  // the problem steps
+    jump = live && lane < TILE && i < td.hi - 1 && jump_here;
+  }
+  const unsigned long long pos = block_append<MARK_BLK>(jump, a.jcnt + a.jslot);
   if (jump && pos < (unsigned long long)a.jcap) {
     a.jlist[pos] = i;
     a.jprob[pos] = p;
@@ -380,7 +380,8 @@ void launch_batch(hes_batch* h, const BatchArgs& a) {
     // jump faces first (a light scan), then their path terms over a fixed
     // grid of at most 1024 waves, then the step
     hipStream_t st = h->stream;
-    hipLaunchKernelGGL(k_bmark2, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_bmark2, dim3((unsigned)((tiles + MARK_BLK / 64 - 1) / (MARK_BLK / 64))),
+                       dim3(MARK_BLK), 0, st, a);
     const unsigned nbl = (unsigned)std::min<long long>(((long long)tiles * TILE + 254) / 256, 1024);
     if (h->flux != HES_FLUX_HLL) {
       if (h->fast) hipLaunchKernelGGL(k_bpath2<1>, dim3(nbl), dim3(64), 0, st, a);

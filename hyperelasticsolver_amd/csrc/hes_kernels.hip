@@ -141,23 +141,18 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : HES_STEP2_
 // jcnt[step & 1]; the other slot is cleared here for the next step.  A step
 // that is a no-op (t >= t_end) also clears its own slot: the host does not
 // count it, so the next run starts at the same step index and slot.  One face
-// per thread, a ballot and one atomic per wave.
-__global__ __launch_bounds__(BLOCK) void k_mark2(StepArgs a) {
-  const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const bool live = a.tring[a.step & 1] < a.t_end;
+// per thread, one atomic per block of MARK_BLK threads.
+constexpr int MARK_BLK = 1024;
+__global__ __launch_bounds__(MARK_BLK) void k_mark2(StepArgs a) {
+  const long long i = (long long)blockIdx.x * MARK_BLK + threadIdx.x;
+  const bool live = a.tring[a.step & 1] < a.t_end;  // uniform over the grid
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.jcnt[(a.step + 1) & 1] = 0ull;
     if (!live) a.jcnt[a.step & 1] = 0ull;
   }
-  if (!live) return;  // the step is a no-op too
+  if (!live) return;  // the step is a no-op too (every thread of every block)
   const bool jump = i < a.n - 1 && face_jump2(a.Qin, i, a.pitch);
-  const unsigned long long m = __ballot(jump);
-  if (m == 0ull) return;
-  unsigned long long base = 0ull;
-  if (lane == 0) base = atomicAdd(a.jcnt + (a.step & 1), (unsigned long long)__popcll(m));
-  base = __shfl(base, 0);
-  const unsigned long long pos = base + __popcll(m & ((1ull << lane) - 1ull));
+  const unsigned long long pos = block_append<MARK_BLK>(jump, a.jcnt + (a.step & 1));
   if (jump && pos < (unsigned long long)a.n) a.jlist[pos] = i;  // at most n - 1 faces
 }
 
@@ -500,8 +495,8 @@ void launch_step_t(const StepArgs& a, long long nchunks, bool fast, hipStream_t 
   if (NPH == 2) {
     // jump faces first (a light scan), then their path terms over a fixed
     // grid of at most one wave per SIMD
-    const unsigned nbp = (unsigned)((a.n + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_mark2, dim3(nbp), dim3(BLOCK), 0, s, a);
+    const unsigned nbp = (unsigned)((a.n + MARK_BLK - 1) / MARK_BLK);
+    hipLaunchKernelGGL(k_mark2, dim3(nbp), dim3(MARK_BLK), 0, s, a);
     const unsigned nbl = (unsigned)std::min<long long>((a.n + 254) / 256, 1024);  // 4 faces per block
     if (FLUX == FLUX_RUSANOV) {
       if (fast) hipLaunchKernelGGL(k_path2<1>, dim3(nbl), dim3(64), 0, s, a);

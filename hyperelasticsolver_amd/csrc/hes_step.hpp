@@ -424,6 +424,33 @@ __device__ __forceinline__ bool face_jump2(const double* __restrict__ Q, long lo
   return !(((a0n - a0) == 0.0) & ((a1n - a1) == 0.0));
 }
 
+// Offset of this lane's entry in a list that every wave of a block of NT
+// threads appends to: one atomic per block on *count (waves that append into
+// one counter one by one serialise at its cache line), the waves' shares in
+// block order, the lanes' in lane order.  Every thread of the block must call
+// it.  Returns the position of a flagged lane's entry.
+template <int NT>
+__device__ __forceinline__ unsigned long long block_append(bool flag, unsigned long long* count) {
+  __shared__ unsigned long long share[NT / 64];
+  __shared__ unsigned long long base;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const unsigned long long m = __ballot(flag);
+  if (lane == 0) share[w] = (unsigned long long)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long total = 0ull;
+#pragma unroll
+    for (int k = 0; k < NT / 64; ++k) {
+      const unsigned long long c = share[k];
+      share[k] = total;
+      total += c;
+    }
+    base = total ? atomicAdd(count, total) : 0ull;
+  }
+  __syncthreads();
+  return base + share[w] + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+}
+
 // One quadrature node's share of a path integral (NumFluxes.jl:39-47, 97-107):
 // w b^p(psi) da_p in the rows of phase block p, where psi is the state at the
 // node, b^p the non-zero column of B's block p and da_p the volume-fraction
