@@ -142,7 +142,7 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : HES_STEP2_
 // that is a no-op (t >= t_end) also clears its own slot: the host does not
 // count it, so the next run starts at the same step index and slot.  One face
 // per thread, one atomic per block of MARK_BLK threads.
-constexpr int MARK_BLK = 1024;
+constexpr int MARK_BLK = 256;
 __global__ __launch_bounds__(MARK_BLK) void k_mark2(StepArgs a) {
   const long long i = (long long)blockIdx.x * MARK_BLK + threadIdx.x;
   const bool live = a.tring[a.step & 1] < a.t_end;  // uniform over the grid
