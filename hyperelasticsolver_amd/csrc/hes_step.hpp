@@ -312,6 +312,11 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
              (i < a.hi - 1);
     }
     auto stream = [&](int r, double fr) {
+      // one register holds the flux for both its uses (shifted to the left
+      // neighbour and local): left to itself the compiler may form the two
+      // from separate copies of the flux arithmetic, contracted differently,
+      // and a uniform state then moves by an ulp (F(Q) - F(Q) must be 0)
+      pin(fr);
       const double J = lane_down(fr) - fr;
       const double dq = lane_down(q[r]) - q[r];
       const double dm = hc.cl * J + hc.clr * dq;

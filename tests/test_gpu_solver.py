@@ -155,14 +155,18 @@ def test_t_end_stops_like_reference(ora):
 
 @pytest.mark.parametrize("nph", [1, 2])
 def test_uniform_state_is_bit_invariant(ora, nph):
-    """F(Q) - F(Q) = 0 exactly: a uniform state never changes (a known answer of the scheme)."""
+    """F(Q) - F(Q) = 0 exactly: a uniform state never changes (a known answer of
+    the scheme).  Eight random states per phase count: whether the flux of a
+    cell and the flux shifted from its neighbour come out of one register
+    shows only for some states (an ulp in one component)."""
     print(f"[uniform] nph={nph}", flush=True)
-    mats, _, Qs, _ = _problem(ora, nph, 10, 1, seed=3)
-    Q0 = np.asfortranarray(np.repeat(Qs[:, :1], 130, axis=1))
-    for flux in ("rusanov", "hll", "lxf"):
-        Qg, r = _gpu_run(nph, mats, flux, Q0, 7)
-        assert r["steps"] == 7
-        assert np.array_equal(Qg, Q0)
+    mats, _, Qs, _ = _problem(ora, nph, 10, 8, seed=3)
+    for k in range(Qs.shape[1]):
+        Q0 = np.asfortranarray(np.repeat(Qs[:, k:k + 1], 130, axis=1))
+        for flux in ("rusanov", "hll", "lxf"):
+            Qg, r = _gpu_run(nph, mats, flux, Q0, 7)
+            assert r["steps"] == 7
+            assert np.array_equal(Qg, Q0), (k, flux)
 
 
 def test_conservation_telescopes(ora):
@@ -338,6 +342,47 @@ def test_two_phase_every_face_jumps(ora, flux):
     print(f"  max per-component rel err {err.max():.3e}", flush=True)
     assert np.all(err <= TOL), err
     np.testing.assert_allclose(r["dt"], dto, rtol=TOL, atol=0)
+
+
+@pytest.mark.parametrize("nph,flux,log2n", [(1, "rusanov", 24), (1, "hll", 24), (1, "hll", 20),
+                                            (2, "hll", 22), (2, "rusanov", 22)])
+def test_largest_grids_conserve_and_leave_the_far_field_alone(ora, nph, flux, log2n):
+    """The largest grids of the benchmarks and beyond (2^24 cells single phase,
+    2^22 two phase; 270 thousand waves per step), 65 pieces with equal end
+    states, 8 steps: the conserved totals change by round-off only (each
+    phase's mass and the mixture rows for two phases), and every cell more
+    than 9 cells from a piece boundary keeps its bits (information moves one
+    cell per step)."""
+    steps, npieces = 8, 65
+    nx = 1 << log2n
+    print(f"[scale] nph={nph} flux={flux} nx=2^{log2n} steps={steps}", flush=True)
+    rng = np.random.default_rng(77 + log2n)
+    if nph == 1:
+        mats = MAT_CU.copy()
+        Ps = _rand_phase(rng, mats, npieces - 1)
+        rows = [[r] for r in range(14)]
+    else:
+        mats = np.concatenate([MAT_CU, MAT_AL])
+        Ps = _rand_two_phase(rng, mats, npieces - 1)
+        rows = [[1], [16], [2, 17], [3, 18], [4, 19], [5, 20]]
+    Qs = ora.prim2cons(nph, mats, Ps)
+    Qs = np.asfortranarray(np.concatenate([Qs, Qs[:, :1]], axis=1))  # equal end states
+    ifc = np.sort(rng.uniform(0.01, 0.99, npieces - 1))
+    Q0 = _piecewise(nx, Qs, ifc)
+    Qg, r = _gpu_run(nph, mats, flux, Q0, steps)
+    assert r["steps"] == steps and np.all(np.isfinite(Qg))
+    for rr in rows:
+        t0, t1 = Q0[rr].sum(), Qg[rr].sum()
+        assert abs(t1 - t0) <= 1e-12 * np.abs(Q0[rr]).sum(), (rr, t1 - t0)
+    idx = np.arange(nx)
+    piece = np.searchsorted(ifc, (idx + 0.5) / nx, side="left")
+    edges = np.flatnonzero(np.diff(piece)) + 0.5  # between cells k and k + 1
+    pos = np.searchsorted(edges, idx)
+    dist = np.minimum(np.abs(idx - edges[np.clip(pos - 1, 0, None)]),
+                      np.abs(edges[np.clip(pos, None, len(edges) - 1)] - idx))
+    far = dist > steps + 1
+    assert far.sum() > nx // 2
+    assert np.array_equal(Qg[:, far], Q0[:, far])
 
 
 @pytest.mark.parametrize("flux", ["rusanov", "hll"])
