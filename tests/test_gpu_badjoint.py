@@ -89,6 +89,37 @@ def test_batch_matches_one_problem_adjoints():
         assert np.abs(gmb - gma).max() <= 1e-9 * scale, (p, gmb, gma)
 
 
+def test_batch_with_its_own_dx_per_problem():
+    """Per-problem dx (here half and twice 1 / n_p): each problem still evolves
+    and differentiates as the one-problem adjoint with that dx."""
+    from hyperelasticsolver_amd.adjoint import Adjoint, BatchAdjoint
+    print("[badjoint] per-problem dx", flush=True)
+    steps, every = 30, 8
+    ncells, pieces = [70, 200, 333], [2, 3, 2]
+    eos, prims, ifcs, rho_t, u_t = _batch_problem(ncells, pieces, steps, seed=35)
+    dx = np.array([0.5 / ncells[0], 2.0 / ncells[1], 1.0 / ncells[2]])
+    with BatchAdjoint(eos, ncells, steps, every=every, dx=dx) as b:
+        b.set_initial(prims, ifcs)
+        b.set_target(rho_t, u_t)
+        L = b.forward()
+        Q = b.final_state()
+        grads = b.backward()
+    off = np.concatenate([[0], np.cumsum(ncells)])
+    for p, n in enumerate(ncells):
+        sl = slice(off[p], off[p + 1])
+        with Adjoint(eos[p], n, steps, every=every, dx=dx[p]) as a:
+            a.set_initial(prims[p], list(ifcs[p]))
+            a.set_target(rho_t[sl], u_t[sl])
+            La = a.forward()
+            Qa = a.final_state()
+            ga, gma = a.backward()
+        assert np.array_equal(Q[:, sl], Qa), f"problem {p}: forward state"
+        assert L[p] == pytest.approx(La, rel=1e-12, abs=1e-300)
+        scale = max(np.abs(ga).max(), np.abs(gma).max())
+        assert np.abs(grads[p][0] - ga).max() <= 1e-9 * scale
+        assert np.abs(grads[p][1] - gma).max() <= 1e-9 * scale
+
+
 def test_batch_checkpoints_skipping_and_launch_change_nothing():
     print("[badjoint] checkpoint interval, skipping and launch invariance", flush=True)
     steps = 45
