@@ -125,12 +125,14 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : HES_STEP2_
   const long long wave = (xcd_block() * BLOCK + threadIdx.x) >> 6;
   const long long i = wave * TILE + lane - 1;
   double tcur;
+  prio_loads();
   if (!step_begin(a, tcur)) return;
   const ChunkCtx ctx = {a.mat, a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, 0, a.n, a.cfl, a.dx,
                         a.relax, a.dpath, a.lxf};
   double q[NC], cc[NPH];
   load_chunk<NPH>(ctx, i, q, cc);
   const double lmax = step_lmax(a);
+  prio_compute();
   constexpr bool PB = NPH == 2;  // path integrals from k_path2 / k_path2_hll
   double lam = chunk_step<NPH, FLUX, FAST, PB, PIN != 0>(ctx, i, q, cc, lmax);
   step_epilogue(a, lam, tcur, lmax);

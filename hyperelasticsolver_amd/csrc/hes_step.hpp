@@ -103,6 +103,15 @@ __device__ __forceinline__ long long xcd_block() {
 // component at a time.
 __device__ __forceinline__ void pin(double& v) { asm volatile("" : "+v"(v)); }
 
+// Wave priority while a wave issues its state loads, and back to the default
+// for its arithmetic: a newly started wave's loads then go out ahead of the
+// FP64 work of the waves already computing, which keeps more loads in flight
+// (single-phase step at 2^22 cells: 205 -> 200 us; two-phase HLL at 2^21:
+// about 301 -> 292 us).  Used by the solver's step only: the batched step,
+// whose waves first read their tile record, was 13% slower with it.
+__device__ __forceinline__ void prio_loads() { __builtin_amdgcn_s_setprio(2); }
+__device__ __forceinline__ void prio_compute() { __builtin_amdgcn_s_setprio(0); }
+
 
 // One 64-lane chunk: lane l holds cell i = first + l - 1 (lanes 0 and 63 are
 // halo cells); q and cc are its state and sound speeds.  Updates the owned
