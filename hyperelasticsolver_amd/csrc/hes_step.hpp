@@ -107,8 +107,8 @@ __device__ __forceinline__ void pin(double& v) { asm volatile("" : "+v"(v)); }
 // for its arithmetic: a newly started wave's loads then go out ahead of the
 // FP64 work of the waves already computing, which keeps more loads in flight
 // (single-phase step at 2^22 cells: 205 -> 200 us; two-phase HLL at 2^21:
-// about 301 -> 292 us).  Used by the solver's step only: the batched step,
-// whose waves first read their tile record, was 13% slower with it.
+// about 301 -> 292 us).  Used by the solver's step only: the batched step
+// was 13% slower with it (0.82 -> 0.93 ms), wherever it was raised.
 __device__ __forceinline__ void prio_loads() { __builtin_amdgcn_s_setprio(2); }
 __device__ __forceinline__ void prio_compute() { __builtin_amdgcn_s_setprio(0); }
 
