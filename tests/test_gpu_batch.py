@@ -115,6 +115,31 @@ def test_batch_over_two_to_the_21_cells(ora, flux, fk):
         assert so == steps and np.all(err <= TOL), (p, err.max())
 
 
+def test_large_problems_in_a_batch_equal_single_problem_runs(ora):
+    """Problems above 65536 cells max their CFL speeds into the slot only when
+    the slot holds a smaller one; smaller problems always issue the atomic.
+    Problems on both sides of that switch must step exactly as they do alone."""
+    from hyperelasticsolver_amd.solver import Solver
+    ncells = [300001, 70, 65537, 65536, 65535]
+    steps = 6
+    print(f"[batch] large problems {ncells}", flush=True)
+    b, eos, sts, ifs = _build(ora, ncells, seed=12)
+    Q0 = b.get_state()
+    b.run(steps, mode="graph")
+    Qb = b.get_state()
+    tb, sb = b.times()
+    offs = list(b.offsets)
+    b.close()
+    for p, n in enumerate(ncells):
+        with Solver(eos[p], n, flux="rusanov", cfl=0.6) as s:
+            s.set_state(Q0[:, offs[p]:offs[p + 1]])
+            r = s.run(steps=steps, use_graph=False)
+            Qs = s.get_state()
+        assert sb[p] == steps == r["steps"]
+        assert tb[p] == r["t"], p
+        assert np.array_equal(Qb[:, offs[p]:offs[p + 1]], Qs), f"problem {p}"
+
+
 def test_batch_matches_oracle(ora):
     from hyperelasticsolver_amd.solver import materials_array
     print("[batch] vs oracle", flush=True)
