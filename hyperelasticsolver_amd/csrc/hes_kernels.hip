@@ -122,7 +122,11 @@ __global__ __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : HES_STEP2_WAVES_PER_EU))) void k_step(StepArgs a) {
   constexpr int NC = NComp<NPH>::value;
   const int lane = threadIdx.x & 63;
-  const long long wave = (xcd_block() * BLOCK + threadIdx.x) >> 6;
+  long long wave = (xcd_block() * BLOCK + threadIdx.x) >> 6;
+  // odd steps walk the tiles from the far end: their first loads are the
+  // lines the previous step stored last, which the memory-side cache may
+  // still hold (the result does not depend on the order)
+  if (a.step & 1) wave = (long long)gridDim.x * (BLOCK / 64) - 1 - wave;
   const long long i = wave * TILE + lane - 1;
   double tcur;
   prio_loads();
