@@ -45,6 +45,7 @@ struct BatchArgs {
   long long pitch;
   int tile_begin, tile_end;
   int step;  // ring phase (step index modulo 6)
+  int alt;   // odd steps walk the waves from the far end (launches of <= ALT_MAX_CELLS)
   int xcd;   // deal consecutive blocks to one XCD
   int lxf;   // Rusanov kernels: Lax-Friedrichs face speed dx / dt (main.jl:50)
   // tiles per wave: the grid has nwaves = ceil(tiles / tpw) waves and wave w
@@ -136,7 +137,9 @@ __device__ __forceinline__ void batch_tile_step(const BatchArgs& a, int tile) {
 template <int NPH, int FLUX, int FAST, int BLK, int PIN = 0, int LOOP = 0>
 __global__ __launch_bounds__(BLK)
 __attribute__((amdgpu_waves_per_eu(3))) void k_batch_step(BatchArgs a) {
-  const int w = batch_wave<BLK>(a);
+  int w = batch_wave<BLK>(a);
+  // the same set of waves either way (see ALT_MAX_CELLS)
+  if ((a.step & 1) && a.alt) w = (int)(gridDim.x * (BLK / 64)) - 1 - w;
   if constexpr (!LOOP) {
     const int tile = a.tile_begin + w;
     if (tile >= a.tile_end) return;  // the whole wave (no block-level sync here)
@@ -337,6 +340,7 @@ BatchArgs batch_args(hes_batch* h, long long s, int tb, int te) {
   a.tile_begin = tb;
   a.tile_end = te;
   a.step = (int)(s % 6);
+  a.alt = (long long)(te - tb) * TILE <= ALT_MAX_CELLS;
   a.xcd = h->xcd;
   a.lxf = h->flux == HES_FLUX_LXF;
   a.tpw = h->tpw;

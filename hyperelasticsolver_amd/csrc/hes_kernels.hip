@@ -59,6 +59,7 @@ struct StepArgs {
   long long* jlist;           // two phases: the jump faces of the step (k_mark2)
   int lxf;                    // Rusanov kernels: Lax-Friedrichs face speed dx / dt (main.jl:50)
   int step;                   // step index modulo 6 (ring phase)
+  int alt;                    // odd steps walk the tiles from the far end (n <= ALT_MAX_CELLS)
   int hist_idx;
   MatDev mat[2];
 };
@@ -123,10 +124,8 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : HES_STEP2_
   constexpr int NC = NComp<NPH>::value;
   const int lane = threadIdx.x & 63;
   long long wave = (xcd_block() * BLOCK + threadIdx.x) >> 6;
-  // odd steps walk the tiles from the far end: their first loads are the
-  // lines the previous step stored last, which the memory-side cache may
-  // still hold (the result does not depend on the order)
-  if (a.step & 1) wave = (long long)gridDim.x * (BLOCK / 64) - 1 - wave;
+  // the same set of waves either way (see ALT_MAX_CELLS)
+  if ((a.step & 1) && a.alt) wave = (long long)gridDim.x * (BLOCK / 64) - 1 - wave;
   const long long i = wave * TILE + lane - 1;
   double tcur;
   prio_loads();
@@ -566,6 +565,7 @@ StepArgs make_args(hes_solver* h, long long s, double t_end, double* hist, int h
   a.jlist = h->jlist;
   a.lxf = h->flux == HES_FLUX_LXF;
   a.step = (int)(s % 6);
+  a.alt = h->n <= ALT_MAX_CELLS;
   a.hist_idx = hist_idx;
   a.mat[0] = h->mat[0];
   a.mat[1] = h->mat[1];
