@@ -1,6 +1,6 @@
 """ctypes binding of the CPU oracle (oracle/libhes_oracle.so) for the tests.
 
-Test infrastructure only: the oracle is the checker, never the thing measured.
+Test infrastructure only: the oracle checks results and is never the thing measured.
 """
 import ctypes
 import math
