@@ -389,6 +389,106 @@ def test_hll_reduces_to_textbook_hll(ora):
         np.testing.assert_allclose(dp, Fr - Fh, atol=1e-13 * scale)
 
 
+def _bcols(ora, mats, Q):
+    """The two non-zero columns of noncons_flux(Q) (HyperelasticityMPh.jl:178-231,
+    omega = 0, k = 1/2, beta = 0), as a (2, 15) array, built from the separately
+    pinned cons2prim, stress and temperature: u_I = (u1 + u2)/2,
+    sigma_I = (th2 sigma1 + th1 sigma2)/(th1 + th2), sigma_I(:,1).u_I and the
+    distortion rows rho A_1c u + rho (A^T (u_I - u))_c."""
+    pr = ora.cons2prim(2, mats, Q)
+    sig, th, u, rho, A = [], [], [], [], []
+    for p in range(2):
+        m = mats[8 * p:8 * p + 8]
+        b = pr[15 * p:15 * p + 15]
+        sig.append(mat(ora.stress(m, b[5], b[6:15])))
+        th.append(ora.temperature(m, b[5], ora.finger(b[6:15])))
+        u.append(b[2:5])
+        rho.append(b[1])
+        A.append(mat(b[6:15]))
+    uI = 0.5 * u[0] + 0.5 * u[1]
+    sI = (th[1] * sig[0] + th[0] * sig[1]) / (th[0] + th[1])
+    cols = np.zeros((2, 15))
+    for p in range(2):
+        cols[p, 0] = uI[0]
+        cols[p, 2:5] = sI[:, 0]
+        cols[p, 5] = sI[:, 0] @ uI
+        atdv = A[p].T @ (uI - u[p])
+        for c in range(3):
+            cols[p, 6 + 3 * c:9 + 3 * c] = rho[p] * A[p][0, c] * u[p]
+            cols[p, 6 + 3 * c] += rho[p] * atdv[c]
+    return cols
+
+
+def _b_int(ora, mats, Qa, Qb):
+    """Gauss-Legendre-6 integral of B(psi) dpsi along psi(s) = Qa (1 - s) + Qb s on
+    [0, 1] (NumFluxes.jl:94-107), nodes and weights from numpy's leggauss."""
+    x, w = np.polynomial.legendre.leggauss(6)
+    out = np.zeros(30)
+    for xi, wi in zip((x + 1.0) / 2.0, w / 2.0):
+        cols = _bcols(ora, mats, Qa * (1.0 - xi) + Qb * xi)
+        for p in range(2):
+            out[15 * p:15 * p + 15] += wi * cols[p] * (Qb[15 * p] - Qa[15 * p])
+    return out
+
+
+def test_two_phase_hll_across_a_volume_fraction_jump(ora):
+    """Two-phase HLL where the volume fractions jump and every other primitive
+    differs across the face too, so B varies along the path (NumFluxes.jl:85-132,
+    the flux main.jl:225 runs on test case 6).  D-, D+ are rebuilt here from
+    pinned pieces: the flux, B's column from the pinned stress and temperature,
+    Gauss-Legendre nodes from numpy, the speed bounds from the pinned
+    eigenvalues, Q_hll = (Q_r s_r - Q_l s_l - (B_int(Q_l, Q_r) + F_r - F_l)) /
+    (s_r - s_l) (:111), and the two-segment integral
+    B_int(Q_l, Q_hll) + B_int(Q_hll, Q_r) (:128-129).  The oracle must agree to
+    1e-12, and D- + D+ must equal F_r - F_l plus the integral along
+    l -> hll -> r.  The faces are chosen so that the two-segment integral differs
+    from the straight one by more than 1e-3 relative; the test shows that the
+    plausible mistakes -- B_int(Q_l, Q_r) in place of the two-segment sum,
+    s_l and s_r swapped in Q_hll, Q_hll without its path term -- each miss the
+    oracle by far more than the tolerance."""
+    rng = np.random.default_rng(61)
+    for case in range(6):
+        PL, PR = _rand_prims(rng, 2, MAT2), _rand_prims(rng, 2, MAT2)
+        PL[0] = rng.uniform(0.15, 0.45)
+        PR[0] = rng.uniform(0.55, 0.85)
+        PL[15], PR[15] = 1.0 - PL[0], 1.0 - PR[0]
+        QL, QR = ora.prim2cons(2, MAT2, PL), ora.prim2cons(2, MAT2, PR)
+        el, er = ora.get_eigvals(2, MAT2, QL), ora.get_eigvals(2, MAT2, QR)
+        em = ora.get_eigvals(2, MAT2, 0.5 * (QL + QR))
+        sl = min(0.0, em.min(), el.min())
+        sr = max(0.0, em.max(), er.max())
+        FL, FR = ora.flux(2, MAT2, QL), ora.flux(2, MAT2, QR)
+
+        def build(sl_h, sr_h, with_path=True, two_segments=True):
+            pint = _b_int(ora, MAT2, QL, QR) if with_path else 0.0
+            Qh = (QR * sr_h - QL * sl_h - (pint + FR - FL)) / (sr_h - sl_h)
+            if two_segments:
+                I = _b_int(ora, MAT2, QL, Qh) + _b_int(ora, MAT2, Qh, QR)
+            else:
+                I = _b_int(ora, MAT2, QL, QR)
+            J = FR - FL + I
+            d = sr - sl
+            dm = -sl / d * J + sl * sr / d * (QR - QL)
+            dp = sr / d * J - sl * sr / d * (QR - QL)
+            return dm, dp, J, I
+
+        dm, dp, J, I2 = build(sl, sr)
+        cons, odm, odp = ora.hll(2, MAT2, QL, QR, el, er)
+        assert np.all(cons == 0.0)  # the conservative part is zeros(30) (NumFluxes.jl:82)
+        scale = max(np.abs(dm).max(), np.abs(dp).max())
+        tol = 1e-12 * scale
+        assert np.abs(odm - dm).max() <= tol, (case, np.abs(odm - dm).max() / scale)
+        assert np.abs(odp - dp).max() <= tol, (case, np.abs(odp - dp).max() / scale)
+        assert np.abs((odm + odp) - J).max() <= tol, case
+        # the two-segment path matters at these faces
+        I1 = _b_int(ora, MAT2, QL, QR)
+        assert np.abs(I2 - I1).max() >= 1e-3 * np.abs(I2).max(), case
+        # planted mistakes miss by far more than the tolerance
+        for wrong in (build(sl, sr, two_segments=False), build(sr, sl), build(sl, sr, with_path=False)):
+            miss = max(np.abs(odm - wrong[0]).max(), np.abs(odp - wrong[1]).max())
+            assert miss >= 1e3 * tol, (case, miss / scale)
+
+
 def test_quadrature_rules(ora):
     x, w = ora.quadrature(0)
     xr, wr = np.polynomial.legendre.leggauss(6)
