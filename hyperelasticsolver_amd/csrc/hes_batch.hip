@@ -100,7 +100,22 @@ __device__ __forceinline__ void batch_tile_step(const BatchArgs& a, int tile) {
   load_chunk<NPH>(ctx, i, q, cc);  // in flight while the problem's time is read
   const double tcur = a.tring[2 * p + (a.step & 1)];
   if (!(tcur < a.t_end[p])) {
-    if (lead && lane == 0) a.tring[2 * p + ((a.step + 1) & 1)] = tcur;
+    // The problem has reached its end time and does not step, but the launch
+    // still flips the buffers: its state, sound speeds, time and CFL maximum
+    // go into the other buffer and slot unchanged, so the current state of
+    // every problem is in the same buffer, and a later larger end time
+    // resumes from a valid CFL maximum.
+    if (lane >= 1 && lane <= TILE && i < hi) {
+#pragma unroll
+      for (int r = 0; r < NC; ++r) st_stream(q[r], a.Qout + r * a.pitch + i);
+#pragma unroll
+      for (int k = 0; k < NPH; ++k) st_stream(cc[k], a.Cout + k * a.pitch + i);
+    }
+    if (lead && lane == 0) {
+      a.tring[2 * p + ((a.step + 1) & 1)] = tcur;
+      a.slots[3 * p + ((a.step + 1) % 3)] = a.slots[3 * p + (a.step % 3)];
+      a.slots[3 * p + ((a.step + 2) % 3)] = 0ull;
+    }
     return;
   }
   const double lmax = __longlong_as_double(*(const long long*)(a.slots + 3 * p + (a.step % 3)));
@@ -741,21 +756,26 @@ int hes_batch_autotune(hes_batch* h, int steps, int* best_block, int* best_xcd, 
   double *q0 = nullptr, *c0 = nullptr, *t0 = nullptr;
   unsigned long long* sl0 = nullptr;
   int* ps0 = nullptr;
-  HIP_TRY(hipMalloc(&q0, qb));
-  HIP_TRY(hipMalloc(&c0, cb));
-  HIP_TRY(hipMalloc(&t0, sizeof(double) * 2 * h->P));
-  HIP_TRY(hipMalloc(&sl0, sizeof(unsigned long long) * 3 * h->P));
-  HIP_TRY(hipMalloc(&ps0, sizeof(int) * h->P));
   const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
-  HIP_TRY(copy_on(h->stream, q0, h->Q[s0 & 1], qb, dd));
-  HIP_TRY(copy_on(h->stream, c0, h->C[s0 & 1], cb, dd));
-  HIP_TRY(copy_on(h->stream, t0, h->tring, sizeof(double) * 2 * h->P, dd));
-  HIP_TRY(copy_on(h->stream, sl0, h->slots, sizeof(unsigned long long) * 3 * h->P, dd));
-  HIP_TRY(copy_on(h->stream, ps0, h->psteps, sizeof(int) * h->P, dd));
+  // every buffer is freed on every path: the first failure skips the rest
+  hipError_t e = hipMalloc(&q0, qb);
+  if (e == hipSuccess) e = hipMalloc(&c0, cb);
+  if (e == hipSuccess) e = hipMalloc(&t0, sizeof(double) * 2 * h->P);
+  if (e == hipSuccess) e = hipMalloc(&sl0, sizeof(unsigned long long) * 3 * h->P);
+  if (e == hipSuccess) e = hipMalloc(&ps0, sizeof(int) * h->P);
+  if (e == hipSuccess) e = copy_on(h->stream, q0, h->Q[s0 & 1], qb, dd);
+  if (e == hipSuccess) e = copy_on(h->stream, c0, h->C[s0 & 1], cb, dd);
+  if (e == hipSuccess) e = copy_on(h->stream, t0, h->tring, sizeof(double) * 2 * h->P, dd);
+  if (e == hipSuccess) e = copy_on(h->stream, sl0, h->slots, sizeof(unsigned long long) * 3 * h->P, dd);
+  if (e == hipSuccess) e = copy_on(h->stream, ps0, h->psteps, sizeof(int) * h->P, dd);
   const int blocks[2] = {64, 256};
   const int tpws[3] = {1, 2, 4};
   float best = 1e30f;
   int bb = h->block, bx = h->xcd, bt = h->tpw, rc = HES_OK;
+  if (e != hipSuccess) {
+    g_err = std::string("autotune save failed: ") + hipGetErrorString(e);
+    rc = HES_ERR_HIP;
+  }
   for (int c = 0; c < 12 && rc == HES_OK; ++c) {
     const int bi = c / 6, x = (c / 3) % 2, ti = c % 3;
     h->block = blocks[bi];

@@ -326,3 +326,75 @@ def test_batch_general_exponents_match_oracle(ora):
                               1.0 / n, steps=10)
         err = rel_err_per_component(Qb[:, offs[p]:offs[p + 1]], Qo)
         assert np.all(err <= TOL), (p, err.max())
+
+
+def _pieces(ora, rng, eos, nph, k):
+    """k distinct conservative states (every piece of a problem differs)."""
+    from hyperelasticsolver_amd.solver import materials_array
+    mats = materials_array(eos)
+    P = np.zeros((14 if nph == 1 else 30, k), order="F")
+    for j in range(k):
+        for ph in range(nph):
+            A = np.eye(3) + 0.05 * rng.standard_normal((3, 3))
+            m = mats[8 * ph:8 * ph + 8]
+            blk = [m[0] / np.linalg.det(A), *rng.uniform(-0.5, 0.5, 3), 1e-3, *A.reshape(-1, order="F")]
+            if nph == 1:
+                P[:, j] = blk
+            else:
+                a = rng.uniform(0.01, 0.99) if ph == 0 else 1.0 - P[0, j]
+                P[15 * ph:15 * ph + 15, j] = [a] + blk
+    return ora.prim2cons(nph, mats, P), np.sort(rng.uniform(0.1, 0.9, k - 1))
+
+
+@pytest.mark.parametrize("nph,flux,mode", [(1, "rusanov", "graph"), (1, "hll", "eager"),
+                                           (1, "rusanov", "loop"), (2, "rusanov", "graph")])
+def test_stopped_problems_keep_their_state_and_resume(ora, nph, flux, mode):
+    """A problem that reaches its end time inside a run stops stepping while the
+    batch goes on.  Problems that stop after 1, 2 and 4 of their own steps in a
+    7-step run (an odd number of launches) must come back with the state,
+    time and step count of a one-problem run to the same end time, bit for
+    bit; with the end times lifted, a further 4 launches must continue them
+    as a one-problem run continues (its CFL maximum carried over the stopped
+    launches)."""
+    from hyperelasticsolver_amd.batched import Batch
+    from hyperelasticsolver_amd.solver import Solver
+    print(f"[batch] stopped problems resume, nph={nph} flux={flux} mode={mode}", flush=True)
+    rng = np.random.default_rng(77)
+    ncells = [150, 64, 300, 97]
+    stop_after = [2, 1, 4, None]
+    eos = _eos_list(rng, len(ncells), nph)
+    sts, ifs = zip(*[_pieces(ora, rng, e, nph, 3) for e in eos])
+    # the end time of a problem that stops after k steps: between its times
+    # after k - 1 and k steps (`while t < T` then takes exactly k steps)
+    t_end = []
+    for p, n in enumerate(ncells):
+        if stop_after[p] is None:
+            t_end.append(np.inf)
+            continue
+        with Solver(eos[p], n, flux=flux, cfl=0.6) as s:
+            s.set_piecewise(sts[p], ifs[p])
+            dts = s.run(steps=stop_after[p], record_dt=True, use_graph=False)["dt"]
+        tk = np.concatenate([[0.0], np.cumsum(dts)])
+        t_end.append(0.5 * (tk[-2] + tk[-1]))
+    with Batch(eos, ncells, flux=flux) as b:
+        b.set_piecewise(sts, ifs)
+        Q0 = b.get_state()
+        b.set_t_end(t_end)
+        b.run(7, mode=mode)
+        Q1, (t1, s1) = b.get_state(), b.times()
+        b.set_t_end(None)
+        b.run(4, mode=mode)
+        Q2, (t2, s2) = b.get_state(), b.times()
+        offs = list(b.offsets)
+    for p, n in enumerate(ncells):
+        k = 7 if stop_after[p] is None else stop_after[p]
+        with Solver(eos[p], n, flux=flux, cfl=0.6) as s:
+            s.set_state(Q0[:, offs[p]:offs[p + 1]])
+            r1 = s.run(steps=7, t_end=t_end[p], use_graph=False)
+            S1 = s.get_state()
+            r2 = s.run(steps=4, use_graph=False)
+            S2 = s.get_state()
+        assert r1["steps"] == s1[p] == k and t1[p] == r1["t"], (p, r1, s1[p], t1[p])
+        assert np.array_equal(Q1[:, offs[p]:offs[p + 1]], S1), f"problem {p} after the stop"
+        assert r2["steps"] == 4 and s2[p] == k + 4 and t2[p] == r2["t"], (p, r2, s2[p], t2[p])
+        assert np.array_equal(Q2[:, offs[p]:offs[p + 1]], S2), f"problem {p} after resuming"
