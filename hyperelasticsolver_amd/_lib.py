@@ -100,6 +100,7 @@ _SIGS = {
     "hes_flux": [_i, _dp, _dp, _dp, _ll],
     "hes_get_eigvals": [_i, _dp, _dp, _dp, _dp, _ll],
     "hes_max_speed": [_i, _dp, _dp, _dp, _ll],
+    "hes_max_speed_noalloc": [_i, _dp, _vp, _vp, _dp, _ll],
     "hes_noncons_flux": [_dp, _dp, _dp, _ll],
     "hes_face_flux": [_i, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ll],
     "hes_energy": [_dp, _dp, _dp, _dp, _ll],

@@ -11,7 +11,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libhes.so")
 
-SOURCES = ["hes_kernels.hip", "hes_batch.hip", "hes_adjoint.hip", "hes_badjoint.hip"]
+SOURCES = ["hes_kernels.hip", "hes_batch.hip", "hes_adjoint.hip", "hes_badjoint.hip",
+           "hes_noalloc.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -243,6 +243,20 @@ int hes_get_eigvals(int nphase, const double* materials, const double* Q, const 
 /* maximum(abs.(get_eigvals(eos, Q, [1,0,0]))) per state (main.jl:209-210). */
 int hes_max_speed(int nphase, const double* materials, const double* Q, double* lambda,
                   long long n);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  hes_max_speed
+ * on device arrays: Q_dev (ncomp x n, Julia layout, device memory) gives
+ * lambda_dev[i] (device, n doubles; may be NULL) as hes_max_speed does, and
+ * *lmax (host) receives max_i lambda_i, the CFL maximum of main.jl:204-212
+ * (0 for n = 0).  It allocates nothing: the per-block partial maxima and
+ * status words live in a fixed device array of 4 MB that each GPU the library
+ * runs on holds until the process ends, so n is limited to
+ * HES_NOALLOC_MAX_STATES (larger n: HES_ERR_ARG).  Synchronous, on the null
+ * stream of the current device.  Calls that overlap on one GPU (from several
+ * host threads) share that array and race. */
+#define HES_NOALLOC_MAX_STATES (1LL << 26)
+int hes_max_speed_noalloc(int nphase, const double* materials, const double* Q_dev,
+                          double* lambda_dev, double* lmax, long long n);
 /* HyperelasticityMPh.noncons_flux (HyperelasticityMPh.jl:178-250), two phases:
  * dense 30x30 column-major per state (n x 900 doubles). */
 int hes_noncons_flux(const double* materials, const double* Q, double* B, long long n);
