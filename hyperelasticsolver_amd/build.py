@@ -22,6 +22,10 @@ HIPCC_FLAGS = [
     "-shared",
     # never emit packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds are fused only within one source expression: every kernel
+    # instantiation that shares a routine (pinned or not, solver or batch)
+    # then rounds it the same way, whatever its control flow
+    "-ffp-contract=on",
     "-I" + os.path.join(ROOT, "include"),
     "-I" + CSRC,
 ]

@@ -103,31 +103,13 @@ __device__ __forceinline__ double lmax_of(const unsigned long long* h, long long
 // Sound speed c = sqrt(rho(Omega)) of a symmetric acoustic tensor (O11 O12 O13
 // O22 O23 O33) and dc/dO (off-diagonal entries counted twice, as each stands
 // for two matrix entries).  rho(Omega) is the eigenvalue of largest modulus,
-// found as spectral_radius_sym does; its eigenvector is the largest cross
+// found by sym3_extreme as the forward step finds it; its eigenvector is the largest cross
 // product of two rows of Omega - mu I.
 __device__ __forceinline__ double speed_and_grad(const double O[6], double g[6]) {
   const double a = O[0], b = O[3], c = O[5], d = O[1], e = O[2], f = O[4];
-  const double c2 = a + b + c;
-  const double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
-  const double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
-  const double r1 = fabs(d) + fabs(e), r2 = fabs(d) + fabs(f), r3 = fabs(e) + fabs(f);
-  const double lb = fmin(a - r1, fmin(b - r2, c - r3));
-  const double hd = 0.5 * (b - c);
-  const double lD = 0.5 * (b + c) + sqrt(hd * hd + f * f);
-  const double ww = d * d + e * e;
-  double ub = fmax(a, lD) + sqrt(ww);
-  if (a > lD) ub = fmin(ub, a + ww * rcp(a - lD));
-  double mu = cubic_top_root(c2, c1, c0, ub, lb);
-  double sgn = mu < 0.0 ? -1.0 : 1.0;
-  double rad = fabs(mu);
-  if (lb < -rad) {
-    double bot = cubic_bottom_root(c2, c1, c0, lb, ub);
-    if (fabs(bot) > rad) {
-      mu = bot;
-      rad = fabs(bot);
-      sgn = bot < 0.0 ? -1.0 : 1.0;
-    }
-  }
+  const double mu = sym3_extreme(O);
+  const double sgn = mu < 0.0 ? -1.0 : 1.0;
+  const double rad = fabs(mu);
   const double A = a - mu, B = b - mu, C = c - mu;
   // rows (A d e), (d B f), (e f C)
   double x1 = d * f - e * B, y1 = e * d - A * f, z1 = A * B - d * d;

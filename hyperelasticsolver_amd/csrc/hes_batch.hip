@@ -41,6 +41,10 @@ struct BatchArgs {
   unsigned long long* slots;                 // 3 per problem
   double* tring;                             // 2 per problem
   int* psteps;                               // executed steps per problem
+  // per problem: the launch index after its last step, i.e. its state is in
+  // Q[plaunch & 1], C[plaunch & 1] and its CFL maximum in slots[3p + plaunch % 3]
+  long long* plaunch;
+  long long launch;                          // this launch's step index
   int* status;                               // [0] error bits
   long long pitch;
   int tile_begin, tile_end;
@@ -100,22 +104,10 @@ __device__ __forceinline__ void batch_tile_step(const BatchArgs& a, int tile) {
   load_chunk<NPH>(ctx, i, q, cc);  // in flight while the problem's time is read
   const double tcur = a.tring[2 * p + (a.step & 1)];
   if (!(tcur < a.t_end[p])) {
-    // The problem has reached its end time and does not step, but the launch
-    // still flips the buffers: its state, sound speeds, time and CFL maximum
-    // go into the other buffer and slot unchanged, so the current state of
-    // every problem is in the same buffer, and a later larger end time
-    // resumes from a valid CFL maximum.
-    if (lane >= 1 && lane <= TILE && i < hi) {
-#pragma unroll
-      for (int r = 0; r < NC; ++r) st_stream(q[r], a.Qout + r * a.pitch + i);
-#pragma unroll
-      for (int k = 0; k < NPH; ++k) st_stream(cc[k], a.Cout + k * a.pitch + i);
-    }
-    if (lead && lane == 0) {
-      a.tring[2 * p + ((a.step + 1) & 1)] = tcur;
-      a.slots[3 * p + ((a.step + 1) % 3)] = a.slots[3 * p + (a.step % 3)];
-      a.slots[3 * p + ((a.step + 2) % 3)] = 0ull;
-    }
+    // the problem has reached its end time: it does not step, and its state
+    // stays in the buffer of its last step (plaunch), from where
+    // batch_carry moves it when the host reads it or changes end times
+    if (lead && lane == 0) a.tring[2 * p + ((a.step + 1) & 1)] = tcur;
     return;
   }
   const double lmax = __longlong_as_double(*(const long long*)(a.slots + 3 * p + (a.step % 3)));
@@ -141,6 +133,7 @@ __device__ __forceinline__ void batch_tile_step(const BatchArgs& a, int tile) {
       a.tring[2 * p + ((a.step + 1) & 1)] = tcur + dt;
       a.slots[3 * p + ((a.step + 2) % 3)] = 0ull;
       a.psteps[p] += 1;
+      a.plaunch[p] = a.launch + 1;  // the problem's state now lives in buffer (launch + 1) & 1
     }
   }
 }
@@ -294,6 +287,49 @@ __global__ void k_batch_copy_ends(const long long* __restrict__ offsets, int P, 
   }
 }
 
+// Problems that reached their end time stop stepping while the batch goes on,
+// so each one's state is left in the buffers of its last step (plaunch).
+// Before the host reads the states or changes end times, k_batch_carry copies
+// the owned cells and sound speeds of every such problem whose buffer differs
+// from that of the current launch index `launch` (one wave per tile: Qout and
+// Cout of the arguments are the other buffers, Qin and Cin the current ones),
+// and k_batch_carry_fin then moves its CFL maximum into the current slot and
+// records it as current.  Two launches, so that no tile reads plaunch after
+// its problem's entry changed.
+template <int NPH>
+__global__ __launch_bounds__(64) void k_batch_carry(BatchArgs a) {
+  constexpr int NC = NComp<NPH>::value;
+  const int tile = __builtin_amdgcn_readfirstlane(a.tile_begin + (int)blockIdx.x);
+  if (tile >= a.tile_end) return;
+  const TileDesc td = a.tiles[tile];
+  if (!((a.plaunch[td.p] ^ a.launch) & 1)) return;  // already in the current buffer
+  const int lane = threadIdx.x & 63;
+  const long long i = td.first + lane;
+  if (lane >= TILE || i >= td.hi) return;
+  double* Q = const_cast<double*>(a.Qin);
+  double* C = const_cast<double*>(a.Cin);
+#pragma unroll
+  for (int r = 0; r < NC; ++r) Q[r * a.pitch + i] = a.Qout[r * a.pitch + i];
+#pragma unroll
+  for (int k = 0; k < NPH; ++k) C[k * a.pitch + i] = a.Cout[k * a.pitch + i];
+}
+
+__global__ void k_batch_carry_fin(unsigned long long* slots, long long* plaunch, int P,
+                                  long long launch) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const long long L = plaunch[p];
+  if (L == launch) return;
+  // the problem's last step max-reduced the CFL speeds of its state into
+  // slot L % 3, and nothing has written its slots since
+  const unsigned long long v = slots[3 * p + L % 3];
+  slots[3 * p + 0] = 0ull;
+  slots[3 * p + 1] = 0ull;
+  slots[3 * p + 2] = 0ull;
+  slots[3 * p + launch % 3] = v;
+  plaunch[p] = launch;
+}
+
 __global__ void k_soa_to_aos_b(const double* __restrict__ in, double* __restrict__ out, int nc,
                                long long n, long long pitch) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -321,6 +357,7 @@ struct hes_batch {
   double *cfl, *dx, *t_end, *tring;
   unsigned long long* slots;
   int *psteps, *status;
+  long long* plaunch;  // per problem: the launch index after its last step
   // two phases: path-term buffer and jump-face list (see BatchArgs)
   double* dpath;
   unsigned long long* jcnt;
@@ -350,6 +387,8 @@ BatchArgs batch_args(hes_batch* h, long long s, int tb, int te) {
   a.slots = h->slots;
   a.tring = h->tring;
   a.psteps = h->psteps;
+  a.plaunch = h->plaunch;
+  a.launch = s;
   a.status = h->status;
   a.pitch = h->pitch;
   a.tile_begin = tb;
@@ -431,6 +470,21 @@ void launch_batch(hes_batch* h, const BatchArgs& a) {
   }
 }
 
+// Every problem's state, sound speeds and CFL maximum into the buffers and
+// slot of the current step index (see k_batch_carry).
+int batch_carry(hes_batch* h) {
+  BatchArgs a = batch_args(h, h->step, 0, h->ntiles);
+  if (h->nph == 1)
+    hipLaunchKernelGGL(k_batch_carry<1>, dim3((unsigned)h->ntiles), dim3(64), 0, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_batch_carry<2>, dim3((unsigned)h->ntiles), dim3(64), 0, h->stream, a);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_batch_carry_fin, dim3((unsigned)((h->P + 127) / 128)), dim3(128), 0,
+                     h->stream, h->slots, h->plaunch, h->P, h->step);
+  HIP_TRY(hipGetLastError());
+  return HES_OK;
+}
+
 int batch_init_speeds(hes_batch* h) {
   long long s = h->step;
   HIP_TRY(hipMemsetAsync(h->slots, 0, sizeof(unsigned long long) * 3 * h->P, h->stream));
@@ -456,6 +510,7 @@ int batch_after_set(hes_batch* h, const double* t0) {
   for (int p = 0; p < h->P; ++p) tr[2 * p] = tr[2 * p + 1] = t0 ? t0[p] : 0.0;
   HIP_TRY(copy_on(h->stream, h->tring, tr.data(), sizeof(double) * 2 * h->P, hipMemcpyHostToDevice));
   HIP_TRY(zero_on(h->stream, h->psteps, sizeof(int) * h->P));
+  HIP_TRY(zero_on(h->stream, h->plaunch, sizeof(long long) * h->P));
   HIP_TRY(zero_on(h->stream, h->status, sizeof(int) * 4));
   if (h->jcnt) HIP_TRY(zero_on(h->stream, h->jcnt, 2 * sizeof(unsigned long long)));
   h->jpar = 0;
@@ -473,7 +528,7 @@ int hes_batch_destroy(hes_batch* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* ptrs[] = {h->Q[0], h->Q[1], h->C[0], h->C[1], h->tiles, h->offsets, h->mats, h->cfl,
                   h->dx, h->t_end, h->tring, h->slots, h->psteps, h->status, h->dpath, h->jcnt,
-                  h->jlist, h->jprob};
+                  h->jlist, h->jprob, h->plaunch};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -558,6 +613,7 @@ int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const
     if (e == hipSuccess) e = zero_on(h->stream, h->jcnt, 2 * sizeof(unsigned long long));
   }
   up((void**)&h->psteps, nullptr, sizeof(int) * nproblems);
+  up((void**)&h->plaunch, nullptr, sizeof(long long) * nproblems);
   up((void**)&h->status, nullptr, sizeof(int) * 4);
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);
   if (e == hipSuccess) e = hipEventCreate(&h->ev1);
@@ -583,6 +639,11 @@ int hes_batch_info(hes_batch* h, long long* total_cells, int* ntiles, int* block
 
 int hes_batch_set_t_end(hes_batch* h, const double* t_end) {
   if (!h) return fail(HES_ERR_ARG, "null handle");
+  // a problem that stopped resumes from the current buffers
+  if (h->has_state) {
+    const int rc = batch_carry(h);
+    if (rc != HES_OK) return rc;
+  }
   std::vector<double> v(h->P, INFINITY);
   if (t_end)
     for (int p = 0; p < h->P; ++p) v[p] = t_end[p];
@@ -653,6 +714,8 @@ int hes_batch_set_piecewise(hes_batch* h, const int* nstates, const double* stat
 int hes_batch_get_state(hes_batch* h, double* Q) {
   if (!h || !Q) return fail(HES_ERR_ARG, "null argument");
   if (!h->has_state) return fail(HES_ERR_ARG, "no state set");
+  const int rc = batch_carry(h);  // problems that stopped early into the current buffer
+  if (rc != HES_OK) return rc;
   size_t bytes = sizeof(double) * (size_t)h->nc * h->N;
   double* tmp = nullptr;
   HIP_TRY(hipMalloc(&tmp, bytes));
@@ -752,10 +815,15 @@ int hes_batch_autotune(hes_batch* h, int steps, int* best_block, int* best_xcd, 
   if (!h || steps < 1) return fail(HES_ERR_ARG, "bad argument");
   if (!h->has_state) return fail(HES_ERR_ARG, "no state set");
   long long s0 = h->step;
+  // every problem's state into the current buffers first: the state saved
+  // and restored below is then the whole state
+  const int rcc = batch_carry(h);
+  if (rcc != HES_OK) return rcc;
   size_t qb = sizeof(double) * (size_t)h->nc * h->pitch, cb = sizeof(double) * (size_t)h->nph * h->pitch;
   double *q0 = nullptr, *c0 = nullptr, *t0 = nullptr;
   unsigned long long* sl0 = nullptr;
   int* ps0 = nullptr;
+  long long* pl0 = nullptr;
   const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
   // every buffer is freed on every path: the first failure skips the rest
   hipError_t e = hipMalloc(&q0, qb);
@@ -763,11 +831,13 @@ int hes_batch_autotune(hes_batch* h, int steps, int* best_block, int* best_xcd, 
   if (e == hipSuccess) e = hipMalloc(&t0, sizeof(double) * 2 * h->P);
   if (e == hipSuccess) e = hipMalloc(&sl0, sizeof(unsigned long long) * 3 * h->P);
   if (e == hipSuccess) e = hipMalloc(&ps0, sizeof(int) * h->P);
+  if (e == hipSuccess) e = hipMalloc(&pl0, sizeof(long long) * h->P);
   if (e == hipSuccess) e = copy_on(h->stream, q0, h->Q[s0 & 1], qb, dd);
   if (e == hipSuccess) e = copy_on(h->stream, c0, h->C[s0 & 1], cb, dd);
   if (e == hipSuccess) e = copy_on(h->stream, t0, h->tring, sizeof(double) * 2 * h->P, dd);
   if (e == hipSuccess) e = copy_on(h->stream, sl0, h->slots, sizeof(unsigned long long) * 3 * h->P, dd);
   if (e == hipSuccess) e = copy_on(h->stream, ps0, h->psteps, sizeof(int) * h->P, dd);
+  if (e == hipSuccess) e = copy_on(h->stream, pl0, h->plaunch, sizeof(long long) * h->P, dd);
   const int blocks[2] = {64, 256};
   const int tpws[3] = {1, 2, 4};
   float best = 1e30f;
@@ -791,9 +861,11 @@ int hes_batch_autotune(hes_batch* h, int steps, int* best_block, int* best_xcd, 
         copy_on(h->stream, h->C[s0 & 1], c0, cb, dd) != hipSuccess ||
         copy_on(h->stream, h->tring, t0, sizeof(double) * 2 * h->P, dd) != hipSuccess ||
         copy_on(h->stream, h->slots, sl0, sizeof(unsigned long long) * 3 * h->P, dd) != hipSuccess ||
-        copy_on(h->stream, h->psteps, ps0, sizeof(int) * h->P, dd) != hipSuccess)
+        copy_on(h->stream, h->psteps, ps0, sizeof(int) * h->P, dd) != hipSuccess ||
+        copy_on(h->stream, h->plaunch, pl0, sizeof(long long) * h->P, dd) != hipSuccess)
       rc = fail(HES_ERR_HIP, "autotune restore failed");
   }
+  (void)hipFree(pl0);
   (void)hipFree(q0);
   (void)hipFree(c0);
   (void)hipFree(t0);

@@ -106,11 +106,27 @@ __device__ __forceinline__ double phase_u1(const MatDev& m, double frac, const d
 // HyperelasticityMPh.jl:106-124 (cons2prim) + Strains.jl:26-52 + EquationsOfState.jl:139-156.
 // FAST: 1 = alpha 1, beta 3, gamma 2 (no pow), 0 = general exponents,
 // -1 = decide from m.fast at run time.
+//
+// G = (A A^T)^-1 is formed from P = a rho A without dividing A out:
+// G = den^2 adj(P P^T) / det(P)^2 with den = a rho, and since
+// rho^2 = det(P) / (a^3 rho0) (cons2prim's density), den^2 / det(P)^2 =
+// (1 / (den rho0))^2 / a^2.  Likewise det A = rho0 / rho, so
+// I3 = det(G) = (rho / rho0)^2.  t.A is formed only where a caller reads it.
 template <bool ONE_FRAC, int FAST = -1>
 __device__ __forceinline__ void thermo(const MatDev& m, double frac, const double* q, Thermo& t) {
   const double* P = q + 5;
-  double iden;
-  phase_density<ONE_FRAC>(m, frac, q, t.rho, t.den, iden, t.bad);
+  const double detP = det3cm(P);
+  double ifr = 1.0, rho2;
+  if (ONE_FRAC) {
+    rho2 = detP * m.irho0;
+  } else {
+    ifr = rcp(frac);
+    rho2 = detP * (ifr * ifr * ifr) * m.irho0;
+  }
+  t.bad = !(detP > 0.0);
+  t.rho = sqrt(rho2);
+  t.den = ONE_FRAC ? t.rho : frac * t.rho;
+  const double iden = rcp(t.den);
   t.u[0] = q[1] * iden;
   t.u[1] = q[2] * iden;
   t.u[2] = q[3] * iden;
@@ -118,37 +134,35 @@ __device__ __forceinline__ void thermo(const MatDev& m, double frac, const doubl
   double e = t.E - 0.5 * (t.u[0] * t.u[0] + t.u[1] * t.u[1] + t.u[2] * t.u[2]);
 #pragma unroll
   for (int k = 0; k < 9; ++k) t.A[k] = P[k] * iden;
-  const double* A = t.A;
-  // B = A A^T (symmetric)
-  double B11 = A[0] * A[0] + A[3] * A[3] + A[6] * A[6];
-  double B22 = A[1] * A[1] + A[4] * A[4] + A[7] * A[7];
-  double B33 = A[2] * A[2] + A[5] * A[5] + A[8] * A[8];
-  double B12 = A[0] * A[1] + A[3] * A[4] + A[6] * A[7];
-  double B13 = A[0] * A[2] + A[3] * A[5] + A[6] * A[8];
-  double B23 = A[1] * A[2] + A[4] * A[5] + A[7] * A[8];
-  double C11 = B22 * B33 - B23 * B23;
-  double C12 = B13 * B23 - B12 * B33;
-  double C13 = B12 * B23 - B13 * B22;
-  double detB = B11 * C11 + B12 * C12 + B13 * C13;
-  double idetB = rcp(detB);
-  t.G11 = C11 * idetB;
-  t.G22 = (B11 * B33 - B13 * B13) * idetB;
-  t.G33 = (B11 * B22 - B12 * B12) * idetB;
-  t.G12 = C12 * idetB;
-  t.G13 = C13 * idetB;
-  t.G23 = (B12 * B13 - B11 * B23) * idetB;
+  // P P^T (symmetric) and its adjugate
+  double B11 = P[0] * P[0] + P[3] * P[3] + P[6] * P[6];
+  double B22 = P[1] * P[1] + P[4] * P[4] + P[7] * P[7];
+  double B33 = P[2] * P[2] + P[5] * P[5] + P[8] * P[8];
+  double B12 = P[0] * P[1] + P[3] * P[4] + P[6] * P[7];
+  double B13 = P[0] * P[2] + P[3] * P[5] + P[6] * P[8];
+  double B23 = P[1] * P[2] + P[4] * P[5] + P[7] * P[8];
+  const double ir = iden * m.irho0;
+  const double sc = ONE_FRAC ? ir * ir : (ir * ir) * (ifr * ifr);  // den^2 / det(P)^2
+  t.G11 = (B22 * B33 - B23 * B23) * sc;
+  t.G22 = (B11 * B33 - B13 * B13) * sc;
+  t.G33 = (B11 * B22 - B12 * B12) * sc;
+  t.G12 = (B13 * B23 - B12 * B33) * sc;
+  t.G13 = (B12 * B23 - B13 * B22) * sc;
+  t.G23 = (B12 * B13 - B11 * B23) * sc;
   t.i1 = t.G11 + t.G22 + t.G33;
   double trG2 = t.G11 * t.G11 + t.G22 * t.G22 + t.G33 * t.G33 +
                 2.0 * (t.G12 * t.G12 + t.G13 * t.G13 + t.G23 * t.G23);
   t.i2 = 0.5 * (t.i1 * t.i1 - trG2);
-  t.i3 = idetB;
+  const double s1 = t.rho * m.irho0;  // I3^(1/2) = 1 / det A = rho / rho0
+  t.i3 = s1 * s1;
   double ith;  // 1 / (cv t0 s_g)
   if (FAST == 1 || (FAST == -1 && m.fast)) {
-    // det A = rho0 / rho, so I3^(1/2) = 1/det A = rho / rho0
-    t.sa = t.rho * m.irho0;
-    t.sb = t.i3 * t.sa;
+    t.sa = s1;
+    t.sb = t.i3 * s1;
     t.sg = t.i3;
-    ith = detB * m.icvt0;
+    // 1 / I3 = (rho0 / rho)^2 = (rho0 a iden)^2
+    const double r0i = ONE_FRAC ? m.rho0 * iden : m.rho0 * (frac * iden);
+    ith = (r0i * r0i) * m.icvt0;
   } else {
     double li3 = log(t.i3);
     t.sa = exp(m.half_a * li3);
@@ -369,31 +383,78 @@ __device__ __forceinline__ double cubic_bottom_root(double c2, double c1, double
   return t;
 }
 
-// Spectral radius of a symmetric 3x3 given as O11 O12 O13 O22 O23 O33.
-// HyperelasticityMPh.jl:263 uses sqrt(|eig|); only the largest modulus is
-// ever consumed (main.jl:210, NumFluxes.jl:90-91).
-__device__ __forceinline__ double spectral_radius_sym(const double O[6]) {
-  double a = O[0], b = O[3], c = O[5], d = O[1], e = O[2], f = O[4];
-  double c2 = a + b + c;
-  double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
-  double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
-  double r1 = fabs(d) + fabs(e), r2 = fabs(d) + fabs(f), r3 = fabs(e) + fabs(f);
-  double lb = fmin(a - r1, fmin(b - r2, c - r3));
-  // Upper bound of the top eigenvalue: with O = [a w^T; w D], lmax <= max(a, lD) + |w|
-  // (lD = top eigenvalue of D), and when a > lD the secular equation
-  // lmax = a + w^T (lmax - D)^-1 w with lmax >= a gives lmax <= a + |w|^2 / (a - lD).
-  double hd = 0.5 * (b - c);
-  double lD = 0.5 * (b + c) + sqrt(hd * hd + f * f);
-  double ww = d * d + e * e;
-  double ub = fmax(a, lD) + sqrt(ww);
-  if (a > lD) ub = fmin(ub, a + ww * rcp(a - lD));
-  double top = cubic_top_root(c2, c1, c0, ub, lb);
-  double rad = fabs(top);
-  if (lb < -rad) {
-    double bot = cubic_bottom_root(c2, c1, c0, lb, ub);
-    rad = fmax(rad, fabs(bot));
+// 1/x to about 2^-50 relative: the hardware reciprocal refined by one Newton
+// step, for iterations that correct their own rounding.
+__device__ __forceinline__ double rcp1(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+
+// Eigenvalue of largest modulus of a symmetric 3x3 given as O11 O12 O13 O22
+// O23 O33.  HyperelasticityMPh.jl:263 takes sqrt(|eig|) of all three; only the
+// largest modulus is ever consumed (main.jl:210, NumFluxes.jl:90-91).
+//
+// Write O = [a w^T; w D] with w = (O12, O13) and D the trailing 2x2 block.
+// When a exceeds lDu = max(O22, O33) + |O23|, an upper bound of D's
+// eigenvalues, the top eigenvalue is the root above a of the secular function
+//   g(l) = l - a - w^T (l - D)^-1 w,
+// which increases (g' >= 1) and is concave there, with g(a) <= 0: Newton from
+// l = a rises monotonically onto the root.  Its curvature is of order
+// |w|^2 / (a - lDu)^3, so for the acoustic tensor (a longitudinal entry well
+// above the shear block) two or three steps reach full precision.  A step
+// from l leaves an error of at most |g''(l)| / (2 g'(l)) step^2 <=
+// (p^2 + q^2) / det^2 / (l - lDu) step^2 (p, q, det as below), which the loop
+// tests with no further evaluation.  Otherwise (and for the rare state that
+// does not settle in 24 steps) Newton on the characteristic cubic from an
+// upper bound finds the top eigenvalue.  A bottom eigenvalue of larger
+// modulus is possible only below the Gershgorin bound lb, and then found by
+// Newton from below.
+__device__ __forceinline__ double sym3_extreme(const double O[6]) {
+  const double a = O[0], b = O[3], c = O[5], d = O[1], e = O[2], f = O[4];
+  const double r1 = fabs(d) + fabs(e), r2 = fabs(d) + fabs(f), r3 = fabs(e) + fabs(f);
+  const double lb = fmin(a - r1, fmin(b - r2, c - r3));
+  const double lDu = fmax(b, c) + fabs(f);
+  double top = a;
+  bool done = false;
+  if (a > lDu) {
+    const double ff = f * f, fd = f * d, fe = f * e;
+#pragma unroll 1
+    for (int it = 0; it < 24; ++it) {
+      const double x = top - b, y = top - c;
+      const double det = x * y - ff;       // det(l - D) > 0
+      const double p = y * d + fe, q = x * e + fd;  // adj(l - D) w
+      const double h = (top - a) * det - (d * p + e * q);  // g(l) det
+      const double pq2 = p * p + q * q, det2 = det * det;
+      const double st = -h * det * rcp1(det2 + pq2);  // -g / g'
+      const double lo = top;
+      top += st;
+      if (!(pq2 * (st * st) > 0x1p-52 * lo * (lo - lDu) * det2)) {
+        done = true;
+        break;
+      }
+    }
   }
-  return rad;
+  if (!done) {
+    const double c2 = a + b + c;
+    const double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
+    const double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
+    const double ub = fmax(a + r1, fmax(b + r2, c + r3));  // Gershgorin
+    top = cubic_top_root(c2, c1, c0, ub, lb);
+  }
+  double mu = top;
+  if (lb < -fabs(top)) {
+    const double c2 = a + b + c;
+    const double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
+    const double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
+    const double bot = cubic_bottom_root(c2, c1, c0, lb, fmax(a + r1, fmax(b + r2, c + r3)));
+    if (fabs(bot) > fabs(top)) mu = bot;
+  }
+  return mu;
+}
+
+// Spectral radius of a symmetric 3x3 (O11 O12 O13 O22 O23 O33).
+__device__ __forceinline__ double spectral_radius_sym(const double O[6]) {
+  return fabs(sym3_extreme(O));
 }
 
 // All three eigenvalues of a symmetric 3x3 (descending): the largest by Newton,
@@ -422,9 +483,10 @@ __device__ __forceinline__ double sound_speed(const MatDev& m, const Thermo& t) 
 }
 
 // Physical flux of a phase (HyperelasticityMPh.jl:146-175) into f[14]
-// = flux block without the volume-fraction entry (which is 0).
+// = flux block without the volume-fraction entry (which is 0).  P is the
+// phase's conserved a rho vec(A), so den A = P in the distortion rows.
 __device__ __forceinline__ void phase_flux(const MatDev& m, double frac, const Thermo& t,
-                                           double f[14]) {
+                                           const double* P, double f[14]) {
   double s[3];
   stress_row1(m, t, s);
   double st0 = frac * s[0], st1 = frac * s[1], st2 = frac * s[2];
@@ -435,12 +497,11 @@ __device__ __forceinline__ void phase_flux(const MatDev& m, double frac, const T
   f[2] = du1 * u[1] - st1;
   f[3] = du1 * u[2] - st2;
   f[4] = du1 * t.E - (u[0] * st0 + u[1] * st1 + u[2] * st2);
-  const double* A = t.A;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     f[5 + 3 * c] = 0.0;  // row 1: u1 A_1c - u1 A_1c
-    f[6 + 3 * c] = t.den * (u[0] * A[1 + 3 * c] - u[1] * A[3 * c]);
-    f[7 + 3 * c] = t.den * (u[0] * A[2 + 3 * c] - u[2] * A[3 * c]);
+    f[6 + 3 * c] = u[0] * P[1 + 3 * c] - u[1] * P[3 * c];
+    f[7 + 3 * c] = u[0] * P[2 + 3 * c] - u[2] * P[3 * c];
   }
 }
 

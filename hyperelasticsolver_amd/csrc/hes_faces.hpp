@@ -52,7 +52,7 @@ __device__ __forceinline__ void cell_flux(const MatDev* mat, const double* q, co
   if (NPH == 1) {
     Thermo t;
     thermo<true, FAST>(mat[0], 1.0, q, t);
-    phase_flux(mat[0], 1.0, t, f);
+    phase_flux(mat[0], 1.0, t, q + 5, f);
     smin = t.u[0] - cc[0];
     smax = t.u[0] + cc[0];
     bad = t.bad;
@@ -63,7 +63,7 @@ __device__ __forceinline__ void cell_flux(const MatDev* mat, const double* q, co
       Thermo t;
       thermo<false, FAST>(mat[p], q[15 * p], q + 15 * p + 1, t);
       f[15 * p] = 0.0;
-      phase_flux(mat[p], q[15 * p], t, f + 15 * p + 1);
+      phase_flux(mat[p], q[15 * p], t, q + 15 * p + 6, f + 15 * p + 1);
       double lo = t.u[0] - cc[p], hi = t.u[0] + cc[p];
       smin = p == 0 ? lo : fmin(smin, lo);
       smax = p == 0 ? hi : fmax(smax, hi);

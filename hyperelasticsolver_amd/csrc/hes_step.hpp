@@ -289,7 +289,7 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
       bad |= t.bad;
       double fp[15];
       fp[0] = 0.0;
-      phase_flux(a.mat[p], q[15 * p], t, fp + 1);
+      phase_flux(a.mat[p], q[15 * p], t, q + 15 * p + 6, fp + 1);
 #pragma unroll
       for (int k = 0; k < 15; ++k) {
         const int r = 15 * p + k;
@@ -349,7 +349,7 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
         bad |= t.bad;
         double fp[15];
         fp[0] = 0.0;
-        phase_flux(a.mat[p], q[15 * p], t, fp + 1);
+        phase_flux(a.mat[p], q[15 * p], t, q + 15 * p + 6, fp + 1);
 #pragma unroll
         for (int k = 0; k < 15; ++k) stream(15 * p + k, fp[k]);
       }
