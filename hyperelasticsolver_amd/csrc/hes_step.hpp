@@ -244,7 +244,9 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
 
   // dt is needed only from here on: the CFL maximum is read after the state
   // loads are in flight (main.jl:212, dt/dx of main.jl:225)
-  const double dtdx = (a.cfl * a.dx / lmax) / a.dx;
+  // dt / dx = cfl / lmax, to about an ulp (dt itself, which the time sums,
+  // is formed exactly in the step's epilogue)
+  const double dtdx = a.cfl * rcp(lmax);
 
   double upd[NC];
   if (FLUX == FLUX_RUSANOV && NPH == 1) {
@@ -293,9 +295,11 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
 #pragma unroll
       for (int k = 0; k < 15; ++k) {
         const int r = 15 * p + k;
-        const double qn = lane_down(q[r]), fn = lane_down(fp[k]);
+        // zero-filling shifts (every lane is active): only lane 63's face and
+        // lane 0's update see the zeros, and neither is stored
+        const double qn = lane_down_z(q[r]), fn = lane_down_z(fp[k]);
         const double cons = 0.5 * (fp[k] + fn) - hl * (qn - q[r]);
-        upd[r] = q[r] - dtdx * (cons - lane_up(cons));  // main.jl:59, conservative part
+        upd[r] = q[r] - dtdx * (cons - lane_up_z(cons));  // main.jl:59, conservative part
         pin(upd[r]);
       }
     }
@@ -334,11 +338,13 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
       // from separate copies of the flux arithmetic, contracted differently,
       // and a uniform state then moves by an ulp (F(Q) - F(Q) must be 0)
       pin(fr);
-      const double J = lane_down(fr) - fr;
-      const double dq = lane_down(q[r]) - q[r];
+      // zero-filling shifts (every lane is active here): lane 63's face and
+      // lane 0's update read the zeros, and neither is stored
+      const double J = lane_down_z(fr) - fr;
+      const double dq = lane_down_z(q[r]) - q[r];
       const double dm = hc.cl * J + hc.clr * dq;
       const double dp = hc.cr * J - hc.clr * dq;
-      upd[r] = q[r] - dtdx * (dm + lane_up(dp));  // conservative part is zero (NumFluxes.jl:82)
+      upd[r] = q[r] - dtdx * (dm + lane_up_z(dp));  // conservative part is zero (NumFluxes.jl:82)
       if (PHASED || PIN) pin(upd[r]);
     };
     if (PHASED) {
