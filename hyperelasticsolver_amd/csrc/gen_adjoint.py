@@ -74,6 +74,18 @@ def sqrt(a):
     return a.g.add("sqrt", (a,))
 
 
+def sqrtp(a):
+    """Square root of a positive argument, by the device's refined reciprocal
+    square root (the forward step's sqrt_pos); IEEE sqrt on the host."""
+    return a.g.add("sqrtp", (a,))
+
+
+def rcp(a):
+    """1 / a by the device's refined reciprocal (the forward step's rcp);
+    IEEE division on the host."""
+    return a.g.add("rcp", (a,))
+
+
 def exp(a):
     return a.g.add("exp", (a,))
 
@@ -100,43 +112,45 @@ def constants(b0, c0, rho0, cv, t0):
 
 
 def thermo(Q, m):
-    """Q: [rho, rho u(3), rho E, rho A(9)] -> derived state of the phase."""
+    """Q: [rho, rho u(3), rho E, rho A(9)] -> derived state of the phase, in the
+    form the forward step uses: G = (A A^T)^-1 from P = rho A as
+    adj(P P^T) / (rho rho0)^2, I3 = (rho / rho0)^2."""
     P = Q[5:14]
     detP = det3(P)
-    rho = sqrt(detP * (1.0 / m["rho0"]))
-    iden = 1.0 / rho
+    irho0 = rcp(m["rho0"])
+    rho = sqrtp(detP * irho0)
+    iden = rcp(rho)
     u = [Q[1] * iden, Q[2] * iden, Q[3] * iden]
     E = Q[4] * iden
     e = E - 0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2])
-    A = [p * iden for p in P]
-    B11 = A[0] * A[0] + A[3] * A[3] + A[6] * A[6]
-    B22 = A[1] * A[1] + A[4] * A[4] + A[7] * A[7]
-    B33 = A[2] * A[2] + A[5] * A[5] + A[8] * A[8]
-    B12 = A[0] * A[1] + A[3] * A[4] + A[6] * A[7]
-    B13 = A[0] * A[2] + A[3] * A[5] + A[6] * A[8]
-    B23 = A[1] * A[2] + A[4] * A[5] + A[7] * A[8]
-    C11 = B22 * B33 - B23 * B23
-    C12 = B13 * B23 - B12 * B33
-    C13 = B12 * B23 - B13 * B22
-    detB = B11 * C11 + B12 * C12 + B13 * C13
-    idetB = 1.0 / detB
-    G = dict(G11=C11 * idetB, G22=(B11 * B33 - B13 * B13) * idetB, G33=(B11 * B22 - B12 * B12) * idetB,
-             G12=C12 * idetB, G13=C13 * idetB, G23=(B12 * B13 - B11 * B23) * idetB)
+    B11 = P[0] * P[0] + P[3] * P[3] + P[6] * P[6]
+    B22 = P[1] * P[1] + P[4] * P[4] + P[7] * P[7]
+    B33 = P[2] * P[2] + P[5] * P[5] + P[8] * P[8]
+    B12 = P[0] * P[1] + P[3] * P[4] + P[6] * P[7]
+    B13 = P[0] * P[2] + P[3] * P[5] + P[6] * P[8]
+    B23 = P[1] * P[2] + P[4] * P[5] + P[7] * P[8]
+    ir = iden * irho0
+    sc = ir * ir
+    G = dict(G11=(B22 * B33 - B23 * B23) * sc, G22=(B11 * B33 - B13 * B13) * sc,
+             G33=(B11 * B22 - B12 * B12) * sc, G12=(B13 * B23 - B12 * B33) * sc,
+             G13=(B12 * B23 - B13 * B22) * sc, G23=(B12 * B13 - B11 * B23) * sc)
     i1 = G["G11"] + G["G22"] + G["G33"]
     trG2 = (G["G11"] * G["G11"] + G["G22"] * G["G22"] + G["G33"] * G["G33"] +
             2.0 * (G["G12"] * G["G12"] + G["G13"] * G["G13"] + G["G23"] * G["G23"]))
     i2 = 0.5 * (i1 * i1 - trG2)
-    i3 = idetB
-    sa = rho * (1.0 / m["rho0"])  # I3^(1/2) = 1/det A = rho/rho0
-    sb = i3 * sa
+    s1 = rho * irho0  # I3^(1/2) = 1/det A = rho/rho0
+    i3 = s1 * s1
+    sa = s1
+    sb = i3 * s1
     sg = i3
-    ith = detB * (1.0 / m["cvt0"])
+    r0i = m["rho0"] * iden
+    ith = (r0i * r0i) * rcp(m["cvt0"])
     W = m["hb0sq"] * sb * (i1 * i1 * (1.0 / 3.0) - i2)
     sam1 = sa - 1.0
     Uc = m["hk0a2"] * sam1 * sam1
     x = clamp_min((e - W - Uc) * ith + 1.0, 1e-6)
     Eth = m["cvt0"] * sg * (x - 1.0)
-    t = dict(rho=rho, u=u, E=E, A=A, i1=i1, i2=i2, i3=i3, sa=sa, sb=sb, sg=sg, W=W, x=x, Eth=Eth)
+    t = dict(rho=rho, u=u, E=E, P=P, i1=i1, i2=i2, i3=i3, sa=sa, sb=sb, sg=sg, W=W, x=x, Eth=Eth)
     t.update(G)
     return t
 
@@ -157,15 +171,16 @@ def flux(m, t):
     s0 = m2r * (kap * (G2_11 - i13 * G11) + c3)
     s1 = m2r * (kap * (G2_12 - i13 * G12))
     s2 = m2r * (kap * (G2_13 - i13 * G13))
-    u, A, den = t["u"], t["A"], t["rho"]
+    u, P, den = t["u"], t["P"], t["rho"]
     du1 = den * u[0]
     F = [du1, du1 * u[0] - s0, du1 * u[1] - s1, du1 * u[2] - s2,
          du1 * t["E"] - (u[0] * s0 + u[1] * s1 + u[2] * s2)]
     FA = [None] * 9
     for c in range(3):
         FA[3 * c] = None  # row 1 of the distortion flux is identically zero
-        FA[3 * c + 1] = den * (u[0] * A[1 + 3 * c] - u[1] * A[3 * c])
-        FA[3 * c + 2] = den * (u[0] * A[2 + 3 * c] - u[2] * A[3 * c])
+        # rho (u1 A_kc - u_k A_1c) with rho A = P
+        FA[3 * c + 1] = u[0] * P[1 + 3 * c] - u[1] * P[3 * c]
+        FA[3 * c + 2] = u[0] * P[2 + 3 * c] - u[2] * P[3 * c]
     return F + FA
 
 
@@ -352,6 +367,10 @@ def emit(fname, g, inputs, outputs, fused_split=None, exact=False):
             fwd.append(f"  const double v{n} = -{val(args[0])};")
         elif op == "sqrt":
             fwd.append(f"  const double v{n} = sqrt({val(args[0])});")
+        elif op == "sqrtp":
+            fwd.append(f"  const double v{n} = gen_sqrt({val(args[0])});")
+        elif op == "rcp":
+            fwd.append(f"  const double v{n} = gen_rcp({val(args[0])});")
         elif op == "exp":
             fwd.append(f"  const double v{n} = exp({val(args[0])});")
         elif op == "clampmin":
@@ -388,6 +407,10 @@ def emit(fname, g, inputs, outputs, fused_split=None, exact=False):
             r.append(f"  g{a0.i} -= {gn};")
         elif op == "sqrt":
             r.append(f"  g{a0.i} += {gn} * (0.5 / v{n});")
+        elif op == "sqrtp":
+            r.append(f"  g{a0.i} += {gn} * (0.5 * gen_rcp(v{n}));")
+        elif op == "rcp":
+            if is_var(a0): r.append(f"  g{a0.i} -= {gn} * (v{n} * v{n});")
         elif op == "exp":
             r.append(f"  g{a0.i} += {gn} * v{n};")
         elif op == "clampmin":
@@ -412,17 +435,38 @@ def emit(fname, g, inputs, outputs, fused_split=None, exact=False):
           "\n".join(fwd) + "\n" + "\n".join(init + seeds + rev) + "\n" + "\n".join(rev_out) + "\n}\n")
     text = head + f1 + "\n" + f2
     if fused_split is not None:
+        # fwd_vjp: the primal of the nodes that feed outputs 0..k-1 (and the
+        # outputs among them), `seed_head(out, obar)` for those outputs, which
+        # returns whether the remaining outputs' cotangents may be non-zero;
+        # only then the primal of the nodes that feed only the remaining
+        # outputs, `seed_tail(out, obar)` for them and their reverse sweep;
+        # last the reverse sweep of the first nodes.  No node of the second
+        # group is an ancestor of the first, so its adjoints are complete
+        # before the first group's sweep.
         head_set = _ancestors(g, outputs[:fused_split])
+        nodes_fwd = [n for n in order if g.nodes[n][0] != "const"]
+        fwd_head = [l for l, n in zip(fwd, nodes_fwd) if n in head_set]
+        fwd_tail = [l for l, n in zip(fwd, nodes_fwd) if n not in head_set]
+        head_out = [k for k, o in enumerate(outputs) if o is None or o.i in head_set]
+        tail_out = [k for k in range(nout) if k not in head_out]
         tail_only = [n for n in reversed(order) if n not in head_set]
         rest = [n for n in reversed(order) if n in head_set]
         rev_tail = [r for n in tail_only for r in rev_node(n)]
         rev_rest = [r for n in rest for r in rev_node(n)]
-        f3 = (f"template <class Seed>\n"
-              f"HES_HD inline void {fname}_fwd_vjp(const double* in, double* out, Seed&& seed, "
-              f"double* ibar) {{\n" + "\n".join(fwd) + "\n" + "\n".join(outs_fwd) + "\n"
+        seed_of = lambda ks: [f"  g{outputs[k].i} += obar[{k}];" for k in ks
+                              if outputs[k] is not None and g.nodes[outputs[k].i][0] != "const"]
+        f3 = (f"template <class SeedHead, class SeedTail>\n"
+              f"HES_HD inline void {fname}_fwd_vjp(const double* in, double* out, SeedHead&& seed_head, "
+              f"SeedTail&& seed_tail, double* ibar) {{\n" + "\n".join(fwd_head) + "\n" +
+              "\n".join(outs_fwd[k] for k in head_out) + "\n"
               f"  double obar[{nout}];\n"
-              f"  const bool tail = seed(static_cast<const double*>(out), obar);\n" +
-              "\n".join(init + seeds) + "\n  if (tail) {\n" + "\n".join("  " + r for r in rev_tail) +
+              f"  const bool tail = seed_head(static_cast<const double*>(out), obar);\n" +
+              "\n".join(init + seed_of(head_out)) + "\n  if (tail) {\n" +
+              "\n".join("  " + l for l in fwd_tail) + "\n" +
+              "\n".join("  " + outs_fwd[k] for k in tail_out) + "\n"
+              f"    seed_tail(static_cast<const double*>(out), obar);\n" +
+              "\n".join("  " + l for l in seed_of(tail_out)) + "\n" +
+              "\n".join("  " + r for r in rev_tail) +
               "\n  }\n" + "\n".join(rev_rest) + "\n" + "\n".join(rev_out) + "\n}\n")
         text += "\n" + f3
     return text
@@ -436,6 +480,22 @@ def main(path=os.path.join(HERE, "hes_adjoint_gen.hpp")):
         "#if !defined(__clang__)\n#pragma GCC diagnostic ignored \"-Wunknown-pragmas\"\n#endif\n"
         "#if defined(__HIPCC__)\n#define HES_HD __host__ __device__\n#else\n#define HES_HD\n#endif\n"
         "namespace hes_gen {\nusing std::sqrt;\nusing std::exp;\n"
+        "// 1/x and sqrt(x) as the forward step forms them on the device (refined\n"
+        "// hardware reciprocal and reciprocal square root); IEEE on the host\n"
+        "HES_HD inline double gen_rcp(double x) {\n"
+        "#if defined(__HIP_DEVICE_COMPILE__)\n"
+        "  double r = __builtin_amdgcn_rcp(x);\n"
+        "  r = fma(fma(-x, r, 1.0), r, r);\n"
+        "  return fma(fma(-x, r, 1.0), r, r);\n"
+        "#else\n  return 1.0 / x;\n#endif\n}\n"
+        "HES_HD inline double gen_sqrt(double x) {\n"
+        "#if defined(__HIP_DEVICE_COMPILE__)\n"
+        "  const double y = __builtin_amdgcn_rsq(x);\n"
+        "  double g = x * y, h = 0.5 * y;\n"
+        "  const double e = fma(-h, g, 0.5);\n"
+        "  g = fma(g, e, g);\n  h = fma(h, e, h);\n"
+        "  return fma(fma(-g, g, x), h, g);\n"
+        "#else\n  return sqrt(x);\n#endif\n}\n"
         "constexpr int CELL_IN = 19, CELL_OUT = 21, P2C_IN = 18, P2C_OUT = 14, LOSS_IN = 17, PROF_IN = 15;\n"]
     g, i, o = trace_cell()
     parts.append(emit("cell", g, i, o, fused_split=14))

@@ -6,7 +6,7 @@
 //
 // Forward model, step s -> s+1 (main.jl:204-225 with NumFluxes.jl:25-60):
 //   lam_i  = |u1_i| + c_i                 (max |eigenvalue|, c = sqrt rho(Omega))
-//   lmax   = max_i lam_i,  dtdx = (cfl dx / lmax) / dx
+//   lmax   = max_i lam_i,  dtdx = cfl / lmax (dt = cfl dx / lmax)
 //   G_f    = 1/2 (F_i + F_i+1) - 1/2 max(lam_i, lam_i+1) (Q_i+1 - Q_i)
 //   Q_i   <- Q_i - dtdx (G_i+1/2 - G_i-1/2)      (interior cells; the end cells stay)
 //
@@ -314,14 +314,16 @@ void k_adj_step(AdjStepArgs a) {
       q[r] = ld_stream(a.Q + r * pitch + ic);
       qb[r] = inr ? ld_stream(a.Qb_in + r * pitch + ic) : 0.0;
     }
+    // the state's sound speeds, stored in row 14 of its slot by the forward step
+    const double cs = ld_stream(a.Q + NQ * pitch + ic);
     // lmax term of step s+1 at its argmax cell
     if (hit) {
       double acc = 0.0;
       for (int b = lane; b < a.nb; b += 64) acc += a.dtp_next[b];
       acc = wave_sum(acc);
       const double lm1 = lmax_of(a.lmax, a.s + 1);
-      const double dtdx1 = (a.cfl * a.dx / lm1) / a.dx;
-      const double lbar = -acc * dtdx1 / lm1;
+      const double dtdx1 = a.cfl * rcp(lm1);  // as chunk_step forms dt / dx
+      const double lbar = -acc * dtdx1 * rcp(lm1);
       if (inr && i == is1) {
         const double* J = a.Jlam + (a.s + 1) * JL;
 #pragma unroll
@@ -333,9 +335,9 @@ void k_adj_step(AdjStepArgs a) {
       }
     }
     const bool interior = inr && i > 0 && i < a.n - 1;
-    const double dtdx = (a.cfl * a.dx / lmax_of(a.lmax, a.s)) / a.dx;
+    const double dtdx = a.cfl * rcp(lmax_of(a.lmax, a.s));  // as chunk_step forms it
     double ib[hes_gen::CELL_IN];
-    reverse_chunk(q, qb, interior, owned, 0.5 * dtdx, a.gin, ib, dts);
+    reverse_chunk(q, qb, interior, owned, cs, 0.5 * dtdx, a.gin, ib, dts);
     bool zero = true;
     if (owned) {
 #pragma unroll
@@ -424,8 +426,8 @@ __global__ __launch_bounds__(64) void k_adj_final(
   for (int b = lane; b < nb; b += 64) acc += dtp0[b];
   acc = wave_sum(acc);
   const double lm0 = lmax_of(lmax, 0);
-  const double dtdx0 = (cfl * dx / lm0) / dx;
-  const double lbar = -acc * dtdx0 / lm0;
+  const double dtdx0 = cfl * rcp(lm0);
+  const double lbar = -acc * dtdx0 * rcp(lm0);
   const long long i0 = istar[0];
   int k0 = 0;
   while (k0 + 1 < nst && i0 >= bounds[k0 + 1]) ++k0;
@@ -485,7 +487,7 @@ struct hes_adjoint {
   MatDev mat;
   int nst;
   double* ck = nullptr;    // nck slots of 15 x pitch (Q, then C)
-  double* seg[2] = {nullptr, nullptr};  // two segment buffers of every - 1 slots of 14 x pitch
+  double* seg[2] = {nullptr, nullptr};  // two segment buffers of every - 1 slots of 15 x pitch (Q, C)
   double* fq[2] = {nullptr, nullptr};
   double* fc[2] = {nullptr, nullptr};
   double* qb[2] = {nullptr, nullptr};
@@ -541,7 +543,7 @@ void fwd_slot(hes_adjoint* h, long long s, double** q, double** c) {
 // segment's recomputed states live in segment buffer `buf`.
 double* seg_slot(hes_adjoint* h, long long s, long long s0, int buf) {
   if (s == s0) return ck_q(h, s0 / h->every);
-  return h->seg[buf] + (s - s0 - 1) * 14 * h->pitch;
+  return h->seg[buf] + (s - s0 - 1) * 15 * h->pitch;
 }
 
 AdjFwdArgs fwd_args(hes_adjoint* h, long long s, const double* qi, double* qo, const double* ci,
@@ -630,15 +632,17 @@ hipError_t record_backward(hes_adjoint* h) {
     const long long end = std::min(s0 + h->every, h->nsteps);
     const int buf = (int)(c & 1);
     for (long long s = s0; s + 1 < end; ++s) {
-      const double* ci = s == s0 ? ck_c(h, c) : h->fc[(s - s0) & 1];
-      double* co = h->fc[(s - s0 + 1) & 1];
+      // each recomputed state keeps its sound speeds in row 14 of its slot,
+      // as the checkpoints do: the reverse step reads its face speeds there
+      const double* ci = seg_slot(h, s, s0, buf) + 14 * h->pitch;
+      double* co = seg_slot(h, s + 1, s0, buf) + 14 * h->pitch;
       hipLaunchKernelGGL(k_adj_fwd, dim3((unsigned)h->nb), dim3(ABLK), 0, st2,
                          fwd_args(h, s, seg_slot(h, s, s0, buf), seg_slot(h, s + 1, s0, buf), ci,
                                   co, false));
     }
     const int cnt = (int)(end - s0);
     hipLaunchKernelGGL(k_adj_lam, dim3((cnt + 63) / 64), dim3(64), 0, st2, ck_q(h, c),
-                       h->seg[buf], 14 * h->pitch, h->pitch, h->istar, h->gin, s0, cnt, h->n,
+                       h->seg[buf], 15 * h->pitch, h->pitch, h->istar, h->gin, s0, cnt, h->n,
                        h->Jlam);
     return hipEventRecord(h->ev_ready[buf], st2);
   };
@@ -752,7 +756,7 @@ int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* materi
     if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
   if (e == hipSuccess) e = dalloc(&h->ck, (size_t)h->nck * 15 * P);
   for (int b = 0; b < 2 && e == hipSuccess && h->every > 1; ++b)
-    e = dalloc(&h->seg[b], (size_t)(h->every - 1) * 14 * P);
+    e = dalloc(&h->seg[b], (size_t)(h->every - 1) * 15 * P);
   for (int b = 0; b < 2 && e == hipSuccess; ++b) {
     e = dalloc(&h->fq[b], 14 * P);
     if (e == hipSuccess) e = dalloc(&h->fc[b], P);
@@ -782,7 +786,7 @@ int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* materi
     std::vector<double*> sl;
     for (long long c = 1; c < h->nck; ++c) sl.push_back(ck_q(h, c));
     for (int b = 0; b < 2; ++b)
-      for (long long j = 0; j + 1 < h->every; ++j) sl.push_back(h->seg[b] + j * 14 * P);
+      for (long long j = 0; j + 1 < h->every; ++j) sl.push_back(h->seg[b] + j * 15 * P);
     sl.push_back(h->fq[0]);
     sl.push_back(h->fq[1]);
     h->nslots = (int)sl.size();
@@ -1008,7 +1012,7 @@ int hes_adjoint_get_profiles(hes_adjoint* h, double* rho, double* u1) {
 int hes_adjoint_info(hes_adjoint* h, long long* device_bytes, long long* checkpoints) {
   if (!h) return fail(HES_ERR_ARG, "null handle");
   const long long P = h->pitch;
-  long long b = h->nck * 15 * P + 2 * (h->every - 1) * 14 * P + 2 * (14 + 1 + 14) * P;
+  long long b = h->nck * 15 * P + 2 * (h->every - 1) * 15 * P + 2 * (14 + 1 + 14) * P;
   b = b * 8 + (h->nsteps + 1) * 8 + h->nsteps * (long long)h->nb * 16 + h->nsteps * (8 + JL * 8);
   if (device_bytes) *device_bytes = b;
   if (checkpoints) *checkpoints = h->nck;

@@ -121,7 +121,7 @@ __device__ __forceinline__ double speed_and_grad(const double O[6], double g[6])
   double vx = x1, vy = y1, vz = z1, nn = n1;
   if (n2 > nn) { vx = x2; vy = y2; vz = z2; nn = n2; }
   if (n3 > nn) { vx = x3; vy = y3; vz = z3; nn = n3; }
-  const double cs = sqrt(rad);
+  const double cs = rad > 0.0 ? sqrt_pos(rad) : 0.0;  // as sound_speed forms it
   // dc/dmu = sgn / (2c), dmu/dO = v v^T / |v|^2
   const double k = nn > 0.0 ? sgn * 0.5 * rcp(cs) * rcp(nn) : 0.0;
   g[0] = k * vx * vx;
@@ -152,6 +152,8 @@ __device__ __forceinline__ void wave_argmax(double& v, long long& idx) {
 //   qb       in: Qbar of state s+1 with its lmax term; out: the face part of
 //            Qbar of state s
 //   interior / owned  as chunk_step defines them
+//   c        the stored sound speed of the lane's cell in state s (as the
+//            forward step read it: the face speeds below are its own)
 //   hdt      dtdx_s / 2
 //   gin      (b0, c0, rho0, cv, t0)
 //   ib       out: the cell-VJP part (ib[0..13] of Qbar_s, ib[14], ib[15] the
@@ -162,7 +164,7 @@ __device__ __forceinline__ void wave_argmax(double& v, long long& idx) {
 // and the face speed s_f = max(lam_i, lam_i+1) passes its cotangent
 // -dtdx/2 g_f . (Q_i+1 - Q_i) to the larger side (the left one on ties).
 __device__ __forceinline__ void reverse_chunk(const double* q, double* qb, bool interior,
-                                              bool owned, double hdt, const double* gin,
+                                              bool owned, double c, double hdt, const double* gin,
                                               double* ib, double& dts) {
   double in[hes_gen::CELL_IN];
 #pragma unroll
@@ -172,10 +174,11 @@ __device__ __forceinline__ void reverse_chunk(const double* q, double* qb, bool 
   double out[hes_gen::CELL_OUT];
 #pragma unroll
   for (int r = 0; r < hes_gen::CELL_IN; ++r) ib[r] = 0.0;
-  // seeds of the cell VJP from the primal (flux F, acoustic tensor, u1)
+  // seeds of the cell VJP from the primal: first those of the flux F and u1;
+  // the acoustic tensor is formed, and its seeds set, only in waves where a
+  // face speed has a cotangent (lbar != 0 on some lane)
+  double lbar = 0.0;
   auto seed = [&](const double* o, double* ob) -> bool {
-    double gO[6];
-    const double c = speed_and_grad(o + NQ, gO);
     const double u1 = o[20];
     const double lam = fabs(u1) + c;
     const double lam_r = lane_down(lam), lam_l = lane_up(lam);
@@ -194,14 +197,19 @@ __device__ __forceinline__ void reverse_chunk(const double* q, double* qb, bool 
       sdL += gL * (q[r] - ql);
       qb[r] += hdt * (sR * gR - sL * gL);
     }
-    const double lbar = (lam >= lam_r ? -hdt * sdR : 0.0) + (lam > lam_l ? -hdt * sdL : 0.0);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) ob[NQ + k] = lbar * gO[k];
+    lbar = (lam >= lam_r ? -hdt * sdR : 0.0) + (lam > lam_l ? -hdt * sdL : 0.0);
     ob[20] = u1 < 0.0 ? -lbar : lbar;
     dts = owned ? 0.5 * fdot - 0.5 * sR * sdR : 0.0;
     return __any(lbar != 0.0);
   };
-  hes_gen::cell_fwd_vjp(in, out, seed, ib);
+  auto seed_acoustic = [&](const double* o, double* ob) {
+    // d lam / d Omega = d c / d Omega (the top eigenvector's outer product)
+    double gO[6];
+    (void)speed_and_grad(o + NQ, gO);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ob[NQ + k] = lbar * gO[k];
+  };
+  hes_gen::cell_fwd_vjp(in, out, seed, seed_acoustic, ib);
 }
 
 }  // namespace hes_adj

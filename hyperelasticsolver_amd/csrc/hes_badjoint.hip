@@ -374,7 +374,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) void k_
     }
   }
   double dts = 0.0, th0 = 0.0, th1 = 0.0;
-  const double dxp = a.dx[p];
   const double* gin = a.gin + 5 * p;
   const long long ic = i < lo ? lo : (i >= hi ? hi - 1 : i);
   double q[NQ], qb[NQ];
@@ -383,14 +382,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) void k_
     q[r] = ld_stream(a.Q + r * pitch + ic);
     qb[r] = inr ? ld_stream(a.Qb_in + r * pitch + ic) : 0.0;
   }
+  // the state's sound speeds, stored in row 14 of its slot by the forward step
+  const double cs = ld_stream(a.Q + NQ * pitch + ic);
   // lmax term of step s+1 at its argmax cell
   if (hit) {
     double acc = 0.0;
     for (int b = a.tfirst[p] + lane; b < a.tfirst[p + 1]; b += 64) acc += a.dtp_next[b];
     acc = wave_sum(acc);
     const double lm1 = lmax_of(a.lmax, (a.s + 1) * a.P + p);
-    const double dtdx1 = (a.cfl * dxp / lm1) / dxp;
-    const double lbar = -acc * dtdx1 / lm1;
+    const double dtdx1 = a.cfl * rcp(lm1);  // as chunk_step forms dt / dx
+    const double lbar = -acc * dtdx1 * rcp(lm1);
     if (inr && i == is1) {
       const double* J = a.Jlam + ((a.s + 1) * a.P + p) * JL;
 #pragma unroll
@@ -402,9 +403,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) void k_
     }
   }
   const bool interior = inr && i > lo && i < hi - 1;
-  const double dtdx = (a.cfl * dxp / lmax_of(a.lmax, a.s * a.P + p)) / dxp;
+  const double dtdx = a.cfl * rcp(lmax_of(a.lmax, a.s * a.P + p));  // as chunk_step forms it
   double ib[hes_gen::CELL_IN];
-  reverse_chunk(q, qb, interior, owned, 0.5 * dtdx, gin, ib, dts);
+  reverse_chunk(q, qb, interior, owned, cs, 0.5 * dtdx, gin, ib, dts);
   bool zero = true;
   if (owned) {
 #pragma unroll
@@ -483,8 +484,8 @@ __global__ __launch_bounds__(64) void k_badj_final(
   tb0 = wave_sum(tb0);
   tc0 = wave_sum(tc0);
   const double lm0 = lmax_of(lmax, p);
-  const double dtdx0 = (cfl * dx[p] / lm0) / dx[p];
-  const double lbar = -acc * dtdx0 / lm0;
+  const double dtdx0 = cfl * rcp(lm0);
+  const double lbar = -acc * dtdx0 * rcp(lm0);
   const double* J0 = Jlam + (long long)p * JL;  // state 0, problem p
   const long long i0 = istar[p];
   double pb0 = 0.0, pc0 = 0.0;
@@ -587,7 +588,7 @@ void bfwd_slot(hes_badjoint* h, long long s, double** q, double** c) {
 
 double* bseg_slot(hes_badjoint* h, long long s, long long s0, int buf) {
   if (s == s0) return bck_q(h, s0 / h->every);
-  return h->seg[buf] + (s - s0 - 1) * 14 * h->pitch;
+  return h->seg[buf] + (s - s0 - 1) * 15 * h->pitch;
 }
 
 BAdjFwdArgs bfwd_args(hes_badjoint* h, long long s, const double* qi, double* qo, const double* ci,
@@ -672,15 +673,17 @@ hipError_t brecord_backward(hes_badjoint* h) {
     const long long end = std::min(s0 + h->every, h->nsteps);
     const int buf = (int)(c & 1);
     for (long long s = s0; s + 1 < end; ++s) {
-      const double* ci = s == s0 ? bck_c(h, c) : h->fc[(s - s0) & 1];
-      double* co = h->fc[(s - s0 + 1) & 1];
+      // each recomputed state keeps its sound speeds in row 14 of its slot,
+      // as the checkpoints do: the reverse step reads its face speeds there
+      const double* ci = bseg_slot(h, s, s0, buf) + 14 * h->pitch;
+      double* co = bseg_slot(h, s + 1, s0, buf) + 14 * h->pitch;
       hipLaunchKernelGGL(k_badj_fwd, dim3(nt), dim3(64), 0, st2,
                          bfwd_args(h, s, bseg_slot(h, s, s0, buf), bseg_slot(h, s + 1, s0, buf), ci,
                                    co, false));
     }
     const int cnt = (int)(end - s0);
     hipLaunchKernelGGL(k_badj_lam, dim3(blocks_of((long long)cnt * h->P, 64)), dim3(64), 0, st2,
-                       bck_q(h, c), h->seg[buf], 14 * h->pitch, h->pitch, h->istar, h->gin, s0, cnt,
+                       bck_q(h, c), h->seg[buf], 15 * h->pitch, h->pitch, h->istar, h->gin, s0, cnt,
                        h->P, h->N, h->Jlam);
     return hipEventRecord(h->ev_ready[buf], st2);
   };
@@ -828,7 +831,7 @@ int hes_badjoint_create(hes_badjoint** out, int nproblems, const long long* ncel
   up(&h->dx, vdx.data(), vdx.size());
   up(&h->gin, gin.data(), gin.size());
   up(&h->ck, nullptr, (size_t)h->nck * 15 * Pt);
-  for (int b = 0; b < 2 && h->every > 1; ++b) up(&h->seg[b], nullptr, (size_t)(h->every - 1) * 14 * Pt);
+  for (int b = 0; b < 2 && h->every > 1; ++b) up(&h->seg[b], nullptr, (size_t)(h->every - 1) * 15 * Pt);
   for (int b = 0; b < 2; ++b) {
     up(&h->fq[b], nullptr, 14 * Pt);
     up(&h->fc[b], nullptr, Pt);
@@ -852,7 +855,7 @@ int hes_badjoint_create(hes_badjoint** out, int nproblems, const long long* ncel
     std::vector<double*> sl;
     for (long long c = 1; c < h->nck; ++c) sl.push_back(bck_q(h, c));
     for (int b = 0; b < 2; ++b)
-      for (long long j = 0; j + 1 < h->every; ++j) sl.push_back(h->seg[b] + j * 14 * Pt);
+      for (long long j = 0; j + 1 < h->every; ++j) sl.push_back(h->seg[b] + j * 15 * Pt);
     sl.push_back(h->fq[0]);
     sl.push_back(h->fq[1]);
     h->nslots = (int)sl.size();
@@ -1102,7 +1105,7 @@ int hes_badjoint_info(hes_badjoint* h, long long* total_cells, long long* device
                       long long* checkpoints) {
   if (!h) return fail(HES_ERR_ARG, "null handle");
   const long long Pt = h->pitch, T = h->ntiles, P = h->P;
-  long long b = h->nck * 15 * Pt + 2 * (h->every - 1) * 14 * Pt + 2 * (14 + 1 + 14) * Pt + 2 * Pt;
+  long long b = h->nck * 15 * Pt + 2 * (h->every - 1) * 15 * Pt + 2 * (14 + 1 + 14) * Pt + 2 * Pt;
   b = b * 8 + (h->nsteps + 1) * P * 8 + h->nsteps * T * 16 + h->nsteps * P * (8 + JL * 8) +
       T * (2 * 4 + 2 * 8 + 2 * 8 + 8);
   if (total_cells) *total_cells = h->N;
