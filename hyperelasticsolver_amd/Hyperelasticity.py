@@ -15,11 +15,9 @@ names, layouts and results of the Julia text:
 - postproc_arrays(eos, Q0) -> (den, ent, vel, strs, eint, info)   Hyperelasticity.jl:180-206
 
 The density of a state is rho = sqrt(det(rho F) / rho0), as cons2prim of the
-multiphase module computes it (HyperelasticityMPh.jl:113-114).  The physics
-(density, entropy, energy, stress, flux) runs on the GPU in libhes's one-phase
-layout [rho, rho u, rho E, rho vec(A)] (vec column-major); this module permutes
-entries between the two layouts and forms e_int = E - |u|^2 / 2 from the
-GPU's rho and u, as Hyperelasticity.jl:32-33 does.
+multiphase module computes it (HyperelasticityMPh.jl:113-114).  The library
+takes the 13-component layout itself (hes_sp_cons2prim, hes_sp_prim2cons,
+hes_sp_flux), so this module only marshals arrays.
 """
 import numpy as np
 
@@ -27,71 +25,42 @@ from ._lib import check, columns, lib, ptr
 from .EquationsOfState import Barton2009
 from .solver import materials_array
 
-# 13-component row-major index of F[i, j] (0-based i, j) and the matching
-# column-major position in the library's 14-component state
-_ROW = [3 + 3 * i + j for i in range(3) for j in range(3)]         # F row-major order
-_COL = [5 + i + 3 * j for i in range(3) for j in range(3)]         # same entries, col-major slot
-
-
 def _mat(eos):
     if not isinstance(eos, Barton2009):
         raise NotImplementedError(f"not implemented for EoS: {type(eos).__name__}")
     return materials_array(eos)
 
 
-def _to14(q13):
-    """13-component states -> [rho, rho u, rho E, rho vec(A)]; rho from det(rho F)."""
-    n = q13.shape[1]
-    out = np.zeros((14, n), order="F")
-    out[1:4] = q13[0:3]
-    out[4] = q13[12]
-    out[_COL] = q13[_ROW]
-    return out
-
-
-def _from14(q14):
-    n = q14.shape[1]
-    out = np.zeros((13, n), order="F")
-    out[0:3] = q14[1:4]
-    out[12] = q14[4]
-    out[_ROW] = q14[_COL]
-    return out
-
-
-def _prims(eos, Q):
+def _sp_prims(eos, Q):
+    """[den, vel (3), vec(F) (9, column-major), e_int] per 13-component state."""
     mats = _mat(eos)
     q, n, was_vec = columns(Q, 13, "Q")
     P = np.empty((14, n), order="F")
-    check(lib().hes_cons2prim(1, ptr(mats), ptr(np.asfortranarray(_to14(q))), ptr(P), n))
-    return mats, q, P, n, was_vec
+    check(lib().hes_sp_cons2prim(ptr(mats), ptr(q), ptr(P), n))
+    return mats, P, n, was_vec
 
 
 def cons2prim(eos, Q):
     """(den, vel, F, e_int) of a 13-component state (Hyperelasticity.jl:25-36);
     for a (13, n) array: den (n,), vel (3, n), F (3, 3, n), e_int (n,)."""
-    mats, q, P, n, was_vec = _prims(eos, Q)
-    den = P[0]
-    vel = P[1:4]
-    F = P[5:14].reshape(3, 3, n, order="F")
-    e_int = q[12] / den - 0.5 * (vel * vel).sum(axis=0)
+    _, P, n, was_vec = _sp_prims(eos, Q)
+    den, vel, F, e_int = P[0], P[1:4], P[4:13].reshape(3, 3, n, order="F"), P[13]
     if was_vec:
         return float(den[0]), vel[:, 0].copy(), F[:, :, 0].copy(), float(e_int[0])
-    return den.copy(), np.asfortranarray(vel), np.asfortranarray(F), e_int
+    return den.copy(), np.asfortranarray(vel), np.asfortranarray(F), e_int.copy()
 
 
 def prim2cons(eos, vel, F, S):
     """13-component state from velocity, the 3x3 distortion F and entropy S
     (Hyperelasticity.jl:70-96): den = rho0 / det F."""
     mats = _mat(eos)
-    F = np.asarray(F, dtype=np.float64).reshape(3, 3)
-    P = np.empty(14)
-    P[0] = mats[0] / np.linalg.det(F)
-    P[1:4] = np.asarray(vel, dtype=np.float64).reshape(3)
-    P[4] = float(S)
-    P[5:14] = F.reshape(-1, order="F")
-    q14 = np.empty(14)
-    check(lib().hes_prim2cons(1, ptr(mats), ptr(P), ptr(q14), 1))
-    return _from14(q14.reshape(14, 1))[:, 0].copy()
+    P = np.empty(13)
+    P[0:3] = np.asarray(vel, dtype=np.float64).reshape(3)
+    P[3:12] = np.asarray(F, dtype=np.float64).reshape(3, 3).reshape(-1, order="F")
+    P[12] = float(S)
+    Q = np.empty(13)
+    check(lib().hes_sp_prim2cons(ptr(mats), ptr(P), ptr(Q), 1))
+    return Q
 
 
 def flux(eos, Q):
@@ -100,9 +69,8 @@ def flux(eos, Q):
      rho (F(3,:) u1 - F(1,:) u3), rho u1 E - u . sigma(1,:)]."""
     mats = _mat(eos)
     q, n, was_vec = columns(Q, 13, "Q")
-    f14 = np.empty((14, n), order="F")
-    check(lib().hes_flux(1, ptr(mats), ptr(np.asfortranarray(_to14(q))), ptr(f14), n))
-    out = _from14(f14)
+    out = np.empty((13, n), order="F")
+    check(lib().hes_sp_flux(ptr(mats), ptr(q), ptr(out), n))
     return out[:, 0].copy() if was_vec else out
 
 
@@ -134,14 +102,18 @@ def postproc_arrays(eos, Q0):
     """Plot arrays of a 13 x nx state (Hyperelasticity.jl:180-206): density,
     entropy, velocity (3, nx), stress (3, 3, nx) with strs[j, k] = sigma[k, j],
     internal energy, and the names tuple."""
-    mats, q, P, n, _ = _prims(eos, Q0)
+    mats, P, n, _ = _sp_prims(eos, Q0)
     den = P[0].copy()
-    ent = P[4].copy()
     vel = np.asfortranarray(P[1:4])
-    eint = q[12] / den - 0.5 * (vel * vel).sum(axis=0)
+    eint = P[13].copy()
+    F = np.asfortranarray(P[4:13])
+    ent = np.empty(n)
+    G = np.empty((9, n), order="F")
+    check(lib().hes_finger(ptr(F), ptr(G), n))
+    check(lib().hes_entropy(ptr(mats), ptr(np.ascontiguousarray(eint)), ptr(G), ptr(ent), n))
     sig = np.empty((9, n), order="F")
     check(lib().hes_stress_den(ptr(mats), ptr(den), ptr(np.ascontiguousarray(eint)),
-                               ptr(np.asfortranarray(P[5:14])), ptr(sig), n))
+                               ptr(F), ptr(sig), n))
     strs = np.transpose(sig.reshape(3, 3, n, order="F"), (1, 0, 2))
     info = ("den", "ent", "vel", "strs", "eint", "info")
     return den, ent, vel, np.asfortranarray(strs), eint, info

@@ -9,7 +9,7 @@ States are 30-vectors or (30, n) arrays; every evaluation runs on the GPU.
 """
 import numpy as np
 
-from ._lib import DomainError, check, columns, lib, ptr
+from ._lib import check, columns, lib, ptr
 from .EquationsOfState import Barton2009, EoS
 from .solver import materials_array
 
@@ -57,37 +57,24 @@ def noncons_flux(eos, Q):
     return out[:, :, 0].copy() if was_vec else out
 
 
-def _phase_to_single(Q15):
-    """[a, a rho, ...] phase block(s) -> (a, one-phase 14-component state)."""
-    x, n, was_vec = columns(Q15, 15, "Q")
-    a = x[0, :].copy()
-    if np.any(a == 0):
-        raise DomainError("zero volume fraction")
-    return a, np.asfortranarray(x[1:, :] / a), was_vec
+def _phase_call(fn, eos, Q, out_w, *extra):
+    if not isinstance(eos, Barton2009):
+        raise NotImplementedError(f"not implemented for EoS: {type(eos).__name__}")
+    x, n, was_vec = columns(Q, 15, "Q")
+    out = np.empty((out_w, n), order="F")
+    check(fn(ptr(eos.params()), ptr(x), *extra, ptr(out), n))
+    return out[:, 0].copy() if was_vec else out
 
 
 def cons2prim(eos, Q):
-    """Per-phase cons -> prim of a 15-component block (HyperelasticityMPh.jl:106-133)."""
-    if not isinstance(eos, Barton2009):
-        raise NotImplementedError(f"not implemented for EoS: {type(eos).__name__}")
-    a, q14, was_vec = _phase_to_single(Q)
-    n = q14.shape[1]
-    out = np.empty((14, n), order="F")
-    check(lib().hes_cons2prim(1, ptr(eos.params()), ptr(q14), ptr(out), n))
-    res = np.vstack([a[None, :], out])
-    return res[:, 0].copy() if was_vec else np.asfortranarray(res)
+    """Per-phase cons -> prim of a 15-component block (HyperelasticityMPh.jl:106-133):
+    [a, rho, u(3), S, vec(A)]."""
+    return _phase_call(lib().hes_phase_cons2prim, eos, Q, 15)
 
 
 def flux(eos, Q):
     """Per-phase physical flux of a 15-component block (HyperelasticityMPh.jl:146-175)."""
-    if not isinstance(eos, Barton2009):
-        raise NotImplementedError(f"not implemented for EoS: {type(eos).__name__}")
-    a, q14, was_vec = _phase_to_single(Q)
-    n = q14.shape[1]
-    out = np.empty((14, n), order="F")
-    check(lib().hes_flux(1, ptr(eos.params()), ptr(q14), ptr(out), n))
-    res = np.vstack([np.zeros((1, n)), out * a[None, :]])
-    return res[:, 0].copy() if was_vec else np.asfortranarray(res)
+    return _phase_call(lib().hes_phase_flux, eos, Q, 15)
 
 
 def get_eigvals(eos, Q, n):
@@ -99,14 +86,8 @@ def get_eigvals(eos, Q, n):
         x, m, was_vec = columns(Q, 30, "Q")
         out = np.empty((12, m), order="F")
         check(lib().hes_get_eigvals(2, ptr(mats), ptr(x), ptr(nv), ptr(out), m))
-    elif isinstance(eos, Barton2009):
-        _, q14, was_vec = _phase_to_single(Q)
-        m = q14.shape[1]
-        out = np.empty((6, m), order="F")
-        check(lib().hes_get_eigvals(1, ptr(eos.params()), ptr(q14), ptr(nv), ptr(out), m))
-    else:
-        raise NotImplementedError(f"not implemented for EoS: {type(eos).__name__}")
-    return out[:, 0].copy() if was_vec else out
+        return out[:, 0].copy() if was_vec else out
+    return _phase_call(lib().hes_phase_get_eigvals, eos, Q, 6, ptr(nv))
 
 
 # Riemann test cases of HyperelasticityMPh.jl:275-426: (a_l1, a_l2, a_r1, a_r2, den,

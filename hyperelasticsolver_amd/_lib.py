@@ -103,6 +103,12 @@ _SIGS = {
     "hes_max_speed_noalloc": [_i, _dp, _vp, _vp, _dp, _ll],
     "hes_noncons_flux": [_dp, _dp, _dp, _ll],
     "hes_face_flux": [_i, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ll],
+    "hes_phase_cons2prim": [_dp, _dp, _dp, _ll],
+    "hes_phase_flux": [_dp, _dp, _dp, _ll],
+    "hes_phase_get_eigvals": [_dp, _dp, _dp, _dp, _ll],
+    "hes_sp_cons2prim": [_dp, _dp, _dp, _ll],
+    "hes_sp_prim2cons": [_dp, _dp, _dp, _ll],
+    "hes_sp_flux": [_dp, _dp, _dp, _ll],
     "hes_energy": [_dp, _dp, _dp, _dp, _ll],
     "hes_entropy": [_dp, _dp, _dp, _dp, _ll],
     "hes_temperature": [_dp, _dp, _dp, _dp, _ll],

@@ -1169,4 +1169,121 @@ int hes_face_flux(int nph, int flux, const double* mats, const double* QL, const
   return HES_OK;
 }
 
+// ---- Hyperelasticity (the single-phase module, Hyperelasticity.jl:25-114):
+// 13-component states [rho u (3), rho F (9, row-major), rho E], converted on
+// the host to and from the library's one-phase layout [rho, rho u, rho E,
+// rho vec(A)] (vec column-major); the physics runs in the point-wise kernels.
+namespace {
+
+inline int sp_index(int i, int j) { return 3 + 3 * i + j; }  // rho F[i, j] in the 13-vector
+
+void sp_to14(const double* q, double* o) {
+  o[0] = 0.0;  // rho: cons2prim forms it from det(rho F) and never reads this slot
+  o[1] = q[0];
+  o[2] = q[1];
+  o[3] = q[2];
+  o[4] = q[12];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) o[5 + i + 3 * j] = q[sp_index(i, j)];
+}
+
+void sp_from14(const double* o, double* q) {
+  q[0] = o[1];
+  q[1] = o[2];
+  q[2] = o[3];
+  q[12] = o[4];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) q[sp_index(i, j)] = o[5 + i + 3 * j];
+}
+
+}  // namespace
+
+int hes_sp_cons2prim(const double* mat, const double* Q, double* P, long long n) {
+  if (!mat || n < 0 || (n > 0 && (!Q || !P))) return fail(HES_ERR_ARG, "null argument");
+  if (n == 0) return HES_OK;
+  std::vector<double> q14((size_t)14 * n), p14((size_t)14 * n);
+  for (long long k = 0; k < n; ++k) sp_to14(Q + 13 * k, q14.data() + 14 * k);
+  const int rc = hes_cons2prim(1, mat, q14.data(), p14.data(), n);
+  if (rc != HES_OK && rc != HES_ERR_DOMAIN) return rc;
+  for (long long k = 0; k < n; ++k) {
+    const double* pk = p14.data() + 14 * k;
+    double* o = P + 14 * k;
+    const double den = pk[0];
+    o[0] = den;
+    for (int r = 0; r < 3; ++r) o[1 + r] = pk[1 + r];
+    for (int r = 0; r < 9; ++r) o[4 + r] = pk[5 + r];  // vec(F), F = A
+    // e_int = E - |u|^2 / 2 with E = rho E / den (Hyperelasticity.jl:32-33)
+    o[13] = Q[13 * k + 12] / den - 0.5 * (pk[1] * pk[1] + pk[2] * pk[2] + pk[3] * pk[3]);
+  }
+  return rc;
+}
+
+int hes_sp_prim2cons(const double* mat, const double* P, double* Q, long long n) {
+  if (!mat || n < 0 || (n > 0 && (!Q || !P))) return fail(HES_ERR_ARG, "null argument");
+  if (n == 0) return HES_OK;
+  std::vector<double> p14((size_t)14 * n), q14((size_t)14 * n);
+  for (long long k = 0; k < n; ++k) {
+    const double* pk = P + 13 * k;  // [vel (3), vec(F) (9), S]
+    double* o = p14.data() + 14 * k;
+    const double* F = pk + 3;
+    const double detF = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[3] * (F[1] * F[8] - F[7] * F[2]) +
+                        F[6] * (F[1] * F[5] - F[4] * F[2]);
+    o[0] = mat[0] / detF;  // den = rho0 / det F (Hyperelasticity.jl:83)
+    for (int r = 0; r < 3; ++r) o[1 + r] = pk[r];
+    o[4] = pk[12];
+    for (int r = 0; r < 9; ++r) o[5 + r] = F[r];
+  }
+  const int rc = hes_prim2cons(1, mat, p14.data(), q14.data(), n);
+  if (rc != HES_OK) return rc;
+  for (long long k = 0; k < n; ++k) sp_from14(q14.data() + 14 * k, Q + 13 * k);
+  return HES_OK;
+}
+
+int hes_sp_flux(const double* mat, const double* Q, double* F, long long n) {
+  if (!mat || n < 0 || (n > 0 && (!Q || !F))) return fail(HES_ERR_ARG, "null argument");
+  if (n == 0) return HES_OK;
+  std::vector<double> q14((size_t)14 * n), f14((size_t)14 * n);
+  for (long long k = 0; k < n; ++k) sp_to14(Q + 13 * k, q14.data() + 14 * k);
+  const int rc = hes_flux(1, mat, q14.data(), f14.data(), n);
+  if (rc != HES_OK && rc != HES_ERR_DOMAIN) return rc;
+  for (long long k = 0; k < n; ++k) sp_from14(f14.data() + 14 * k, F + 13 * k);
+  return rc;
+}
+
+// ---- One phase block of HyperelasticityMPh (15 components with the volume
+// fraction): the per-phase methods cons2prim / flux / get_eigvals(eos::T, Q)
+// (HyperelasticityMPh.jl:106-133, 146-175, 258-266).  A phase's values do
+// not depend on the other phase, so each block is evaluated as both phases
+// of a two-phase state and the first block of the result is returned.
+namespace {
+
+int phase_call(int op, const double* mat, const double* Q, double* out, long long n, int ow,
+               const double* nvec) {
+  if (!mat || n < 0 || (n > 0 && (!Q || !out))) return fail(HES_ERR_ARG, "null argument");
+  if (n == 0) return HES_OK;
+  double mm[16];
+  for (int r = 0; r < 8; ++r) mm[r] = mm[8 + r] = mat[r];
+  std::vector<double> q30((size_t)30 * n), o2((size_t)2 * ow * n);
+  for (long long k = 0; k < n; ++k)
+    for (int r = 0; r < 15; ++r) q30[30 * k + r] = q30[30 * k + 15 + r] = Q[15 * k + r];
+  const int rc = point_call(op, 2, mm, q30.data(), o2.data(), n, 30, 2 * ow, nvec);
+  if (rc != HES_OK && rc != HES_ERR_DOMAIN) return rc;
+  for (long long k = 0; k < n; ++k)
+    for (int r = 0; r < ow; ++r) out[(size_t)ow * k + r] = o2[(size_t)2 * ow * k + r];
+  return rc;
+}
+
+}  // namespace
+
+int hes_phase_cons2prim(const double* mat, const double* Q, double* P, long long n) {
+  return phase_call(OP_CONS2PRIM, mat, Q, P, n, 15, nullptr);
+}
+int hes_phase_flux(const double* mat, const double* Q, double* F, long long n) {
+  return phase_call(OP_FLUX, mat, Q, F, n, 15, nullptr);
+}
+int hes_phase_get_eigvals(const double* mat, const double* Q, const double* nvec, double* eig,
+                          long long n) {
+  return phase_call(OP_EIGVALS, mat, Q, eig, n, 6, nvec);
+}
+
 }  // extern "C"

@@ -269,6 +269,32 @@ int hes_face_flux(int nphase, int flux, const double* materials, const double* Q
                   const double* QR, const double* aux, double* cons, double* dm, double* dp,
                   long long n);
 
+/* One phase block of HyperelasticityMPh (15 components [a, a rho, a rho u(3),
+ * a rho E, a rho vec(A)], one material): the per-phase methods
+ * cons2prim(eos, Q) (HyperelasticityMPh.jl:106-133; P = [a, rho, u(3), S,
+ * vec(A)]), flux(eos, Q) (HyperelasticityMPh.jl:146-175) and
+ * get_eigvals(eos, Q, n) (HyperelasticityMPh.jl:258-266; 6 per state as
+ * hes_get_eigvals, nvec NULL for [1,0,0]). */
+int hes_phase_cons2prim(const double* material, const double* Q, double* P, long long n);
+int hes_phase_flux(const double* material, const double* Q, double* F, long long n);
+int hes_phase_get_eigvals(const double* material, const double* Q, const double* nvec,
+                          double* eig, long long n);
+
+/* Hyperelasticity, the single-phase module (Hyperelasticity.jl:25-114), in its
+ * own 13-component layout Q = [rho u (3), rho F (9, row-major: rho F[i,j] at
+ * 3 i + j, 1-based), rho E], one material.  The Julia module cannot run as
+ * written (it calls density, energy and stress without the equation of
+ * state); these take the material first and keep the formulas, with
+ * den = sqrt(det(rho F) / rho0) as HyperelasticityMPh forms it.
+ *   hes_sp_cons2prim (Hyperelasticity.jl:25-36): P = [den, vel (3), vec(F) (9,
+ *     column-major: the memory of Julia's 3x3 F), e_int], 14 per state;
+ *   hes_sp_prim2cons (Hyperelasticity.jl:70-96): P = [vel (3), vec(F) (9,
+ *     column-major), S], 13 per state, den = rho0 / det F;
+ *   hes_sp_flux (Hyperelasticity.jl:99-114): 13 per state. */
+int hes_sp_cons2prim(const double* material, const double* Q, double* P, long long n);
+int hes_sp_prim2cons(const double* material, const double* P, double* Q, long long n);
+int hes_sp_flux(const double* material, const double* Q, double* F, long long n);
+
 /* EquationsOfState (single material, 8 doubles) */
 /* energy(eos, S, G) (EquationsOfState.jl:118-137); G: 9 x n */
 int hes_energy(const double* material, const double* S, const double* G, double* e, long long n);

@@ -104,10 +104,13 @@ def test_multiphase_module(ora):
         _close(np.sort(ev[:, k]), np.sort(eo[:, k]), 1e-10, "get_eigvals")
     for k in range(0, NPTS, 8):
         _close(H.noncons_flux(eos, Q[:, k]), ora.noncons_flux(mats, Q[:, k]), 1e-10, "noncons")
-        # per-phase methods on a 15-component block
-        blk = Q[:15, k]
-        _close(H.flux(eos[0], blk), ora.flux(2, mats, Q[:, k])[:15], 1e-11, "phase flux")
-        _close(H.cons2prim(eos[0], blk), ora.cons2prim(2, mats, Q[:, k])[:15], 1e-10, "phase c2p")
+        # per-phase methods on each 15-component block (HyperelasticityMPh.jl:106,146,258)
+        for ph in range(2):
+            blk, sl = Q[15 * ph:15 * ph + 15, k], slice(15 * ph, 15 * ph + 15)
+            _close(H.flux(eos[ph], blk), ora.flux(2, mats, Q[:, k])[sl], 1e-11, "phase flux")
+            _close(H.cons2prim(eos[ph], blk), ora.cons2prim(2, mats, Q[:, k])[sl], 1e-10, "phase c2p")
+            _close(np.sort(H.get_eigvals(eos[ph], blk, [1.0, 0.0, 0.0])),
+                   np.sort(ora.get_eigvals(2, mats, Q[:, k])[6 * ph:6 * ph + 6]), 1e-10, "phase eig")
 
 
 def test_multiphase_initial_states(ora):
