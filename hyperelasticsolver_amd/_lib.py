@@ -59,6 +59,7 @@ _SIGS = {
                        ctypes.POINTER(ctypes.c_float), _dp],
     "hes_solver_time_steps": [_vp, _ll, ctypes.POINTER(ctypes.c_float)],
     "hes_solver_set_relaxation": [_vp, _d],
+    "hes_solver_set_pressure_relaxation": [_vp, _d],
     "hes_copy_bandwidth": [_ll, _i, _dp],
     "hes_adjoint_create": [ctypes.POINTER(_vp), _ll, _dp, _d, _d, _ll, _ll],
     "hes_adjoint_destroy": [_vp],

@@ -54,6 +54,7 @@ struct StepArgs {
   long long n, pitch;
   double cfl, dx, t_end;
   double relax;               // two phases: 1 / tau of the velocity relaxation (0: none)
+  double relax_p;             // two phases: 1 / tau of the pressure relaxation (0: none)
   const double* dpath;        // two phases: path terms of the jump faces (k_path2, k_path2_hll)
   unsigned long long* jcnt;   // two phases: jump-face count of step s in jcnt[s & 1]
   long long* jlist;           // two phases: the jump faces of the step (k_mark2)
@@ -118,7 +119,8 @@ __device__ __forceinline__ void step_epilogue(const StepArgs& a, double lam_new,
 }
 
 // One chunk per wave.  PIN: see chunk_step (single phase, Rusanov, large grids).
-template <int NPH, int FLUX, int FAST, int PIN = 0>
+// SRC (two phases): with the relaxation sources (see chunk_step).
+template <int NPH, int FLUX, int FAST, int PIN = 0, int SRC = 0>
 __global__ __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : HES_STEP2_WAVES_PER_EU))) void k_step(StepArgs a) {
   constexpr int NC = NComp<NPH>::value;
@@ -131,13 +133,13 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : HES_STEP2_
   prio_loads();
   if (!step_begin(a, tcur)) return;
   const ChunkCtx ctx = {a.mat, a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, 0, a.n, a.cfl, a.dx,
-                        a.relax, a.dpath, a.lxf};
+                        a.relax, a.dpath, a.lxf, a.relax_p};
   double q[NC], cc[NPH];
   load_chunk<NPH>(ctx, i, q, cc);
   const double lmax = step_lmax(a);
   prio_compute();
   constexpr bool PB = NPH == 2;  // path integrals from k_path2 / k_path2_hll
-  double lam = chunk_step<NPH, FLUX, FAST, PB, PIN != 0>(ctx, i, q, cc, lmax);
+  double lam = chunk_step<NPH, FLUX, FAST, PB, PIN != 0, SRC != 0>(ctx, i, q, cc, lmax);
   step_epilogue(a, lam, tcur, lmax);
 }
 
@@ -474,6 +476,7 @@ struct hes_solver {
   MatDev mat[2];
   double cfl, dx;
   double relax;         // 1 / tau of the two-phase velocity relaxation (0: none)
+  double relax_p;       // 1 / tau of the two-phase pressure relaxation (0: none)
   double* dpath;        // two phases: half path integrals of the faces (ncomp x pitch)
   unsigned long long* jcnt;  // two phases: jump-face counts (2 slots, by step parity)
   long long* jlist;          // two phases: jump-face list (n entries)
@@ -521,6 +524,15 @@ void launch_step_t(const StepArgs& a, long long nchunks, bool fast, hipStream_t 
       return;
     }
   }
+  if constexpr (NPH == 2) {
+    if (a.relax > 0.0 || a.relax_p > 0.0) {  // with the relaxation sources
+      if (fast)
+        hipLaunchKernelGGL((k_step<NPH, FLUX, 1, 0, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_step<NPH, FLUX, 0, 0, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
+      return;
+    }
+  }
   if (fast)
     hipLaunchKernelGGL((k_step<NPH, FLUX, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
   else
@@ -560,6 +572,7 @@ StepArgs make_args(hes_solver* h, long long s, double t_end, double* hist, int h
   a.dx = h->dx;
   a.t_end = t_end;
   a.relax = h->relax;
+  a.relax_p = h->relax_p;
   a.dpath = h->dpath;
   a.jcnt = h->jcnt;
   a.jlist = h->jlist;
@@ -683,6 +696,14 @@ int hes_solver_set_relaxation(hes_solver* h, double tau) {
   if (h->nph != 2) return fail(HES_ERR_ARG, "velocity relaxation needs two phases");
   if (!(tau > 0.0)) return fail(HES_ERR_ARG, "tau must be positive (infinity: off)");
   h->relax = std::isinf(tau) ? 0.0 : 1.0 / tau;
+  return HES_OK;
+}
+
+int hes_solver_set_pressure_relaxation(hes_solver* h, double tau) {
+  if (!h) return fail(HES_ERR_ARG, "null handle");
+  if (h->nph != 2) return fail(HES_ERR_ARG, "pressure relaxation needs two phases");
+  if (!(tau > 0.0)) return fail(HES_ERR_ARG, "tau must be positive (INFINITY: off)");
+  h->relax_p = std::isinf(tau) ? 0.0 : 1.0 / tau;
   return HES_OK;
 }
 

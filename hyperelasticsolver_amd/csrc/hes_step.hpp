@@ -145,6 +145,7 @@ struct ChunkCtx {
   // lambda = 1 / dtdx = dx / dt on every face (main.jl:30-41, 50) in place of
   // max(lambda_i, lambda_i+1); the path integrals do not depend on it
   int lxf = 0;
+  double relax_p = 0.0;  // two phases: 1 / tau of the pressure relaxation (0: none)
 };
 
 // Interphase velocity relaxation of a two-phase cell over a time dt (an
@@ -184,13 +185,127 @@ __device__ __forceinline__ void relax_velocity2(const MatDev* mat, double* q, do
   q[20] += 0.5 * m[1] * (kn[1] - ke[1]) + 0.5 * dk;
 }
 
+// Pressure -tr(sigma)/3 of a phase: with tr sigma = -2 rho (2 W + 3 c3),
+// p = rho (4/3 W + 2 c3) = rho ((4/3 + beta) W + 2 k0/(2a) (s_a - 1) s_a + gamma E_th).
+__device__ __forceinline__ double phase_pressure(const MatDev& m, const Thermo& t) {
+  return t.rho * ((4.0 / 3.0 + m.beta) * t.W + 2.0 * m.k0_2a * (t.sa - 1.0) * t.sa +
+                  m.gamma * t.Eth);
+}
+
+// The pressure of a phase after its volume fraction changes from a to a x at
+// fixed mass, velocity and distortion shape (A -> x^(1/3) A), with its
+// internal energy changed by de: rho -> rho / x, I3^(k/2) -> I3^(k/2) x^-k,
+// W -> W x^-(beta + 4/3) (I1^2/3 - I2 scales as x^-4/3), and E_th from the
+// energy balance e = W + U_cold + E_th (with the entropy law's clamp).
+struct PhaseScale {
+  double rho, W, sa, sg, e;
+};
+template <int FAST>
+__device__ __forceinline__ double scaled_pressure(const MatDev& m, const PhaseScale& s, double x,
+                                                  double de) {
+  const double ix = rcp(x);
+  double fa, fb, fg;  // x^-alpha, x^-beta x^-4/3, x^-gamma
+  if (FAST == 1 || (FAST == -1 && m.fast)) {
+    const double ix2 = ix * ix;
+    fa = ix;
+    fb = ix2 * ix2 * cbrt(ix);
+    fg = ix2;
+  } else {
+    const double lx = log(x);
+    fa = exp(-m.alpha * lx);
+    fb = exp(-(m.beta + 4.0 / 3.0) * lx);
+    fg = exp(-m.gamma * lx);
+  }
+  const double rho = s.rho * ix, sa = s.sa * fa, sg = s.sg * fg, W = s.W * fb;
+  const double sam1 = sa - 1.0;
+  const double eth_min = m.cvt0 * sg * (1e-6 - 1.0);  // x_S >= 1e-6 (EquationsOfState.jl:152-154)
+  const double Eth = fmax(s.e + de - W - m.hk0a2 * sam1 * sam1, eth_min);
+  return rho * ((4.0 / 3.0 + m.beta) * W + 2.0 * m.k0_2a * sam1 * sa + m.gamma * Eth);
+}
+
+// Pressure relaxation of a two-phase cell over a time dt (an optional stiff
+// source; the reference has none), solved implicitly:
+//   a1' - a1 = (dt / tau) (p1' - p2') / Z,  Z = sum_k rho_k c0_k^2 / a_k,
+// with p_k = -tr(sigma_k)/3 at the end of the step.  Z is the linearised
+// stiffness -d(p1 - p2)/da1, so near equilibrium p1 - p2 decays as
+// exp(-t / tau); as tau -> 0 the step ends in pressure equilibrium.  Each
+// phase keeps its mass, momentum and distortion shape (a rho A scales by
+// (a'/a)^(1/3), so rho' = rho a / a'), and the phases exchange the work of
+// the interface pressure p_I = (p1 + p2) / 2 of the start of the step:
+// a_k rho_k E_k changes by -p_I (a_k' - a_k).  Mixture mass, momentum and
+// energy are unchanged, a1 + a2 is kept, and a state in pressure equilibrium
+// is left alone.  The residual g(d) = d - (dt/tau)/Z (p1'(d) - p2'(d))
+// increases with d = a1' - a1 and changes sign on (-a1, a2) (a phase
+// compressed to nothing has unbounded pressure); its root is found by secant
+// steps kept inside a shrinking bracket.
+template <int FAST>
+__device__ __forceinline__ void relax_pressure2(const MatDev* mat, double* q, double dtau) {
+  PhaseScale ps[2];
+  double p[2], Z = 0.0;
+  const double a[2] = {q[0], q[15]};
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    Thermo t;
+    thermo<false, FAST>(mat[k], a[k], q + 15 * k + 1, t);
+    ps[k] = {t.rho, t.W, t.sa, t.sg, t.E - 0.5 * (t.u[0] * t.u[0] + t.u[1] * t.u[1] + t.u[2] * t.u[2])};
+    p[k] = phase_pressure(mat[k], t);
+    Z += t.rho * (mat[k].k0 + (4.0 / 3.0) * mat[k].b0sq) * rcp(a[k]);
+  }
+  const double kc = dtau * rcp(Z);
+  const double pI = 0.5 * (p[0] + p[1]);
+  // de_k = -p_I (a_k' - a_k) / (a_k rho_k), with a_k rho_k from the density
+  const double im0 = rcp(a[0] * ps[0].rho), im1 = rcp(a[1] * ps[1].rho);
+  auto g = [&](double d) {
+    const double p0 = scaled_pressure<FAST>(mat[0], ps[0], (a[0] + d) * rcp(a[0]), -pI * d * im0);
+    const double p1 = scaled_pressure<FAST>(mat[1], ps[1], (a[1] - d) * rcp(a[1]), pI * d * im1);
+    return d - kc * (p0 - p1);
+  };
+  const double g0 = -kc * (p[0] - p[1]);
+  if (g0 == 0.0) return;  // pressure equilibrium: nothing moves
+  // bracket [lo, hi] of the root; first guess from the linearised stiffness
+  double lo = g0 < 0.0 ? 0.0 : -a[0] * (1.0 - 1e-12);
+  double hi = g0 < 0.0 ? a[1] * (1.0 - 1e-12) : 0.0;
+  double d0 = 0.0, gd0 = g0;
+  double d1 = -g0 * rcp(1.0 + kc * Z);
+  d1 = fmin(fmax(d1, lo), hi);
+#pragma unroll 1
+  for (int it = 0; it < 60; ++it) {
+    const double gd1 = g(d1);
+    if (gd1 == 0.0) break;
+    if (gd1 < 0.0) lo = d1;
+    else hi = d1;
+    double dn = d1 - gd1 * (d1 - d0) * rcp(gd1 - gd0);
+    if (!(dn > lo && dn < hi)) dn = 0.5 * (lo + hi);  // outside the bracket: bisect
+    const bool done = fabs(dn - d1) <= 4e-16 * fmax(a[0], a[1]) || hi - lo <= 4e-16;
+    d0 = d1;
+    gd0 = gd1;
+    d1 = dn;
+    if (done) break;
+  }
+  const double d = d1;
+  const double a0n = a[0] + d, a1n = a[1] - d;
+  const double s0 = cbrt(a0n * rcp(a[0])), s1 = cbrt(a1n * rcp(a[1]));
+  q[0] = a0n;
+  q[15] = a1n;
+  q[5] -= pI * d;
+  q[20] += pI * d;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    q[6 + r] *= s0;
+    q[21 + r] *= s1;
+  }
+}
+
 // PIN (single phase): each updated component is pinned where it is computed
 // (see pin): Rusanov 102 instead of 160 VGPRs, HLL 106 instead of 136, four
 // waves per SIMD instead of three.  It pays on large grids (2^22 cells:
 // Rusanov 203 -> 200 us, HLL 242 -> 229 us) and costs on small ones, whose
 // few waves per SIMD want the unpinned schedule's instruction-level
 // parallelism (2^18 cells, Rusanov: 19.6 -> 20.3 us).
-template <int NPH, int FLUX, int FAST, bool PATHBUF = false, bool PIN = false>
+// SRC (two phases): the optional relaxation sources after the flux update
+// (velocity and pressure relaxation, each on when its rate is non-zero); a
+// separate instantiation, so that the plain step carries none of their code.
+template <int NPH, int FLUX, int FAST, bool PATHBUF = false, bool PIN = false, bool SRC = false>
 __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, const double* q,
                                              const double* cc, double lmax,
                                              double* lam_cur = nullptr) {
@@ -421,8 +536,13 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
 
   const bool owned = inr && lane >= 1 && lane <= TILE;
   const bool interior = owned && i > a.lo && i < a.hi - 1;
-  if (NPH == 2 && a.relax > 0.0 && interior)
-    relax_velocity2(a.mat, upd, exp(-(a.cfl * a.dx / lmax) * a.relax));
+  if constexpr (NPH == 2 && SRC) {
+    if (interior) {
+      const double dt = a.cfl * a.dx / lmax;  // main.jl:212
+      if (a.relax > 0.0) relax_velocity2(a.mat, upd, exp(-dt * a.relax));
+      if (a.relax_p > 0.0) relax_pressure2<FAST>(a.mat, upd, dt * a.relax_p);
+    }
+  }
   double lam_new = 0.0;
   int bad2 = 0;
   if (interior) {

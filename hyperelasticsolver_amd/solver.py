@@ -139,6 +139,13 @@ class Solver:
         tau = inf switches it off."""
         check(lib().hes_solver_set_relaxation(self._h, float(tau)))
 
+    def set_pressure_relaxation(self, tau=math.inf):
+        """Two phases: implicit pressure relaxation with time scale tau after
+        every flux update (an optional stiff source; the reference has none):
+        the volume fractions move towards equal phase pressures, conserving
+        mixture mass, momentum and energy.  tau = inf switches it off."""
+        check(lib().hes_solver_set_pressure_relaxation(self._h, float(tau)))
+
     # -- stepping
     def run(self, steps=None, t_end=math.inf, use_graph=True, record_dt=False):
         """Advance up to `steps` steps while t < t_end.  Returns a dict with the

@@ -90,6 +90,18 @@ int hes_solver_run(hes_solver* h, long long max_steps, double t_end, int use_gra
  * tau = INFINITY switches it off (the default). */
 int hes_solver_set_relaxation(hes_solver* h, double tau);
 
+/* Optional stiff source, two phases only (the reference has none): pressure
+ * relaxation with time scale tau, solved implicitly per interior cell after
+ * each flux update (and after the velocity relaxation):
+ *     a1' - a1 = (dt / tau) (p1' - p2') / Z,   Z = sum_k rho_k c0_k^2 / a_k,
+ * p_k = -tr(sigma_k) / 3 of the end state.  Each phase keeps its mass,
+ * momentum and distortion shape (a rho A scales by (a'/a)^(1/3)), and
+ * a_k rho_k E_k changes by -p_I (a_k' - a_k) with p_I the mean pressure of the
+ * start state: mixture mass, momentum and energy are conserved, a1 + a2 is
+ * kept, and as tau -> 0 each step ends in pressure equilibrium.
+ * tau = INFINITY switches it off (the default). */
+int hes_solver_set_pressure_relaxation(hes_solver* h, double tau);
+
 /* Time `reps` back-to-back steps (the step kernels and nothing else) with HIP
  * events on the solver stream; the state advances by reps steps.  The steps
  * are captured into one graph before the first event (as hes_solver_run
