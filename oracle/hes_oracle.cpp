@@ -26,6 +26,13 @@
 // Bit-level agreement with Julia (ForwardDiff op order, LAPACK rounding) is not
 // claimed; agreement is to a few ulps per operation.
 //
+// The single-phase path (EOS, cons2prim/prim2cons, flux, get_eigvals, lxf and
+// the time loop) is templated on its scalar type, so that ForwardDiff.gradient
+// of a misfit through the whole run is restated too (ora_adjoint_dual: 28
+// tangents carried by Dual<double,28> beneath the EOS's nested duals; the
+// eigenvalues of a dual matrix take first-order perturbation partials).  The
+// double instantiations are the same operations as before, bit for bit.
+//
 // Layout: a phase block has 15 components [a, a*r, a*r*u(3), a*r*E, a*r*A(9)],
 // A column-major (HyperelasticityMPh.jl:80-84).  Single-phase states have 14
 // components: the phase block without the volume fraction (a == 1).
@@ -34,6 +41,7 @@
 #include <cstdint>
 #include <cstring>
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -59,6 +67,26 @@ inline double val(const Dual<T, N>& x) { return val(x.v); }
 
 template <class T, int N>
 inline Dual<T, N> lift(const T& c) { return Dual<T, N>(c, 0); }
+
+// lift_to<T, U>::of(u): u (of a scalar type U at some nesting level of T) as a
+// T with zero partials at every level above U.
+template <class T, class U> struct lift_to;
+template <class T> struct lift_to<T, T> {
+  static T of(const T& u) { return u; }
+};
+template <class V, int N, class U> struct lift_to<Dual<V, N>, U> {
+  static Dual<V, N> of(const U& u) { return Dual<V, N>(lift_to<V, U>::of(u), 0); }
+};
+template <class V, int N> struct lift_to<Dual<V, N>, Dual<V, N>> {
+  static Dual<V, N> of(const Dual<V, N>& u) { return u; }
+};
+
+// A material constant inside an expression of type T: a double stays a
+// double (so the double instantiations keep their mixed double operations);
+// a differentiated constant (the parameter tangents of the time-loop duals
+// below) is lifted to T.
+template <class T> inline const double& K(const double& p) { return p; }
+template <class T, class U> inline T K(const U& p) { return lift_to<T, U>::of(p); }
 
 #define DUAL_T template <class T, int N>
 DUAL_T inline Dual<T, N> operator+(const Dual<T, N>& a, const Dual<T, N>& b) {
@@ -252,10 +280,18 @@ inline void invariants(const T* g, T* inv) {
 // ---------------------------------------------------------------------------
 // EquationsOfState.jl:71-156  (Barton2009)
 // ---------------------------------------------------------------------------
-struct Barton2009 {
-  double rho0, c0, cv, t0, b0, alpha, beta, gamma;
-  double b0sq, k0;
+// P: the type of the two differentiable constants (double, or a dual carrying
+// their tangents); rho0, cv, t0 and the exponents are plain numbers.
+template <class P>
+struct BartonT {
+  double rho0;
+  P c0;
+  double cv, t0;
+  P b0;
+  double alpha, beta, gamma;
+  P b0sq, k0;
 };
+using Barton2009 = BartonT<double>;
 
 inline Barton2009 make_barton(const double* p) {  // EquationsOfState.jl:90-115
   Barton2009 e;
@@ -266,30 +302,30 @@ inline Barton2009 make_barton(const double* p) {  // EquationsOfState.jl:90-115
   return e;
 }
 
-template <class T>
-inline T energy(const Barton2009& e, const T& S, const T* G) {  // :118-137
+template <class T, class E>
+inline T energy(const E& e, const T& S, const T* G) {  // :118-137
   T i[3];
   invariants(G, i);
-  T U = 0.5 * e.k0 / (e.alpha * e.alpha) * sq(pow_(i[2], 0.5 * e.alpha) - 1.0) +
+  T U = 0.5 * K<T>(e.k0) / (e.alpha * e.alpha) * sq(pow_(i[2], 0.5 * e.alpha) - 1.0) +
         e.cv * e.t0 * pow_(i[2], 0.5 * e.gamma) * (exp_(S / e.cv) - 1.0);
-  T W = 0.5 * e.b0sq * pow_(i[2], 0.5 * e.beta) * (sq(i[0]) / 3.0 - i[1]);
+  T W = 0.5 * K<T>(e.b0sq) * pow_(i[2], 0.5 * e.beta) * (sq(i[0]) / 3.0 - i[1]);
   return U + W;
 }
 
-template <class T>
-inline T entropy(const Barton2009& e, const T& e_int, const T* G) {  // :139-156
+template <class T, class E>
+inline T entropy(const E& e, const T& e_int, const T* G) {  // :139-156
   T i[3];
   invariants(G, i);
-  T S = e_int - 0.5 * e.b0sq * pow_(i[2], 0.5 * e.beta) * (sq(i[0]) / 3.0 - i[1]) -
-        0.5 * e.k0 / (e.alpha * e.alpha) * sq(pow_(i[2], 0.5 * e.alpha) - 1.0);
+  T S = e_int - 0.5 * K<T>(e.b0sq) * pow_(i[2], 0.5 * e.beta) * (sq(i[0]) / 3.0 - i[1]) -
+        0.5 * K<T>(e.k0) / (e.alpha * e.alpha) * sq(pow_(i[2], 0.5 * e.alpha) - 1.0);
   S = (S / (e.cv * e.t0 * pow_(i[2], 0.5 * e.gamma)) + 1.0);
   if (S < 1e-6) S = T(1e-6);
   return log_(S) * e.cv;
 }
 
 // dedG = gradient(G -> energy(eos, S, G), G) at fixed S, 9 independent entries
-template <class T>
-inline void dedG_grad(const Barton2009& e, const T& S, const T* G, T* dedG) {
+template <class T, class E>
+inline void dedG_grad(const E& e, const T& S, const T* G, T* dedG) {
   typedef Dual<T, 9> D;
   D Gd[9];
   for (int k = 0; k < 9; ++k) {
@@ -301,19 +337,20 @@ inline void dedG_grad(const Barton2009& e, const T& S, const T* G, T* dedG) {
 }
 
 // stress(eos, ent, F) :179-190
-template <class T>
-inline void stress_ent(const Barton2009& e, double ent, const T* F, T* sig) {
+// (ent is held fixed: a double, or a scalar of a lower nesting level of T)
+template <class T, class E, class S>
+inline void stress_ent(const E& e, const S& ent, const T* F, T* sig) {
   T den = e.rho0 / det_lu(F);
   T G[9], dedG[9], mG[9];
   finger(F, G);
-  dedG_grad<T>(e, T(ent), G, dedG);
+  dedG_grad<T>(e, lift_to<T, S>::of(ent), G, dedG);
   for (int k = 0; k < 9; ++k) mG[k] = (-2.0 * den) * G[k];
   matmul3(mG, dedG, sig);
 }
 
 // stress(eos, den, e_int, F) :162-176
-template <class T>
-inline void stress_den(const Barton2009& e, const T& den, const T& e_int, const T* F, T* sig) {
+template <class T, class E>
+inline void stress_den(const E& e, const T& den, const T& e_int, const T* F, T* sig) {
   T G[9], dedG[9], mG[9];
   finger(F, G);
   T S = entropy(e, e_int, G);
@@ -323,19 +360,19 @@ inline void stress_den(const Barton2009& e, const T& den, const T& e_int, const 
 }
 
 // acoustic(eos, ent, F, n) :223-246 ; jacobian of stress w.r.t. F (nested duals)
-inline void acoustic_ent(const Barton2009& e, double ent, const double* F, const double* n,
-                         double* ac) {
-  typedef Dual<double, 9> D;
+template <class T, class E>
+inline void acoustic_ent(const E& e, const T& ent, const T* F, const double* n, T* ac) {
+  typedef Dual<T, 9> D;
   D Fd[9], sd[9];
   for (int k = 0; k < 9; ++k) {
-    Fd[k] = D(F[k]);
-    Fd[k].d[k] = 1.0;
+    Fd[k] = lift<T, 9>(F[k]);
+    Fd[k].d[k] = T(1.0);
   }
   stress_ent<D>(e, ent, Fd, sd);
-  double den = e.rho0 / det_lu(F);
-  double inv_den = 1.0 / den;
+  T den = e.rho0 / det_lu(F);
+  T inv_den = 1.0 / den;
   // dTdF[m,i,j,l] = d sig(m,i) / d F(j,l) = sd[m + 3i].d[j + 3l]
-  for (int k = 0; k < 9; ++k) ac[k] = 0.0;
+  for (int k = 0; k < 9; ++k) ac[k] = T(0.0);
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
       for (int k = 0; k < 3; ++k)
@@ -370,8 +407,12 @@ inline void acoustic_prim(const Barton2009& e, const double* P, const double* n,
 }
 
 // Eigenvalues of a 3x3 whose antisymmetric part is round-off: cyclic Jacobi on
-// the symmetric part.
-inline void eigvals3(const double* a, double* w) {
+// the symmetric part.  V (optional) receives the accumulated rotation, whose
+// column i is the unit eigenvector of w[i].
+inline void eigvals3(const double* a, double* w, double (*V)[3] = nullptr) {
+  if (V)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) V[i][j] = i == j ? 1.0 : 0.0;
   double s[3][3];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) s[i][j] = 0.5 * (M(a, i, j) + M(a, j, i));
@@ -395,25 +436,53 @@ inline void eigvals3(const double* a, double* w) {
           s[p][k] = c * spk - sn * sqk;
           s[q][k] = sn * spk + c * sqk;
         }
+        if (V)
+          for (int k = 0; k < 3; ++k) {  // V <- V J (the column rotation)
+            double vkp = V[k][p], vkq = V[k][q];
+            V[k][p] = c * vkp - sn * vkq;
+            V[k][q] = sn * vkp + c * vkq;
+          }
       }
   }
   w[0] = s[0][0]; w[1] = s[1][1]; w[2] = s[2][2];
 }
 
+// Eigenvalues of a matrix of first-order duals: the values as above, the
+// partials of each simple eigenvalue from first-order perturbation theory,
+// d(w_i) = v_i^T dA v_i with v_i its unit eigenvector (the derivative that
+// forward-mode differentiation through a converged symmetric eigensolver
+// yields).
+template <int N>
+inline void eigvals3(const Dual<double, N>* a, Dual<double, N>* w) {
+  double av[9], wv[3], V[3][3];
+  for (int k = 0; k < 9; ++k) av[k] = a[k].v;
+  eigvals3(av, wv, V);
+  for (int i = 0; i < 3; ++i) {
+    w[i] = Dual<double, N>(wv[i], 0);
+    for (int n = 0; n < N; ++n) {
+      double d = 0.0;
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) d += V[j][i] * V[k][i] * M(a, j, k).d[n];
+      w[i].d[n] = d;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // HyperelasticityMPh.jl:63-266 (per phase, 15 components)
 // ---------------------------------------------------------------------------
-inline void prim2cons_phase(const Barton2009& e, const double* P, double* Q) {  // :66-87
-  double frac = P[0], true_den = P[1];
-  double den = frac * true_den;
-  const double* vel = P + 2;
-  double ent = P[5];
-  const double* def_grad = P + 6;
-  double G[9];
+template <class T, class E>
+inline void prim2cons_phase(const E& e, const T* P, T* Q) {  // :66-87
+  T frac = P[0], true_den = P[1];
+  T den = frac * true_den;
+  const T* vel = P + 2;
+  T ent = P[5];
+  const T* def_grad = P + 6;
+  T G[9];
   finger(def_grad, G);
-  double e_int = energy(e, ent, G);
-  double e_kin = (sq(vel[0]) + sq(vel[1]) + sq(vel[2])) / 2.0;
-  double e_total = e_int + e_kin;
+  T e_int = energy(e, ent, G);
+  T e_kin = (sq(vel[0]) + sq(vel[1]) + sq(vel[2])) / 2.0;
+  T e_total = e_int + e_kin;
   Q[0] = frac;
   Q[1] = den;
   for (int k = 0; k < 3; ++k) Q[2 + k] = den * vel[k];
@@ -421,29 +490,33 @@ inline void prim2cons_phase(const Barton2009& e, const double* P, double* Q) {  
   for (int k = 0; k < 9; ++k) Q[6 + k] = den * def_grad[k];
 }
 
-struct PhasePrims {
-  double frac, true_den, den, vel[3], e_total, e_int, def_grad[9];
+template <class T>
+struct PhasePrimsT {
+  T frac, true_den, den, vel[3], e_total, e_int, def_grad[9];
 };
+using PhasePrims = PhasePrimsT<double>;
 
-inline void prims_phase(const Barton2009& e, const double* Q, PhasePrims& p) {  // :106-121
+template <class T, class E>
+inline void prims_phase(const E& e, const T* Q, PhasePrimsT<T>& p) {  // :106-121
   p.frac = Q[0];
-  double FQ[9];
+  T FQ[9];
   for (int k = 0; k < 9; ++k) FQ[k] = Q[6 + k] / p.frac;
-  p.true_den = std::sqrt(det_lu(FQ) / e.rho0);
+  p.true_den = sqrt_(det_lu(FQ) / e.rho0);
   p.den = p.frac * p.true_den;
   for (int k = 0; k < 3; ++k) p.vel[k] = Q[2 + k] / p.den;
   p.e_total = Q[5] / p.den;
-  double e_kin = (sq(p.vel[0]) + sq(p.vel[1]) + sq(p.vel[2])) / 2.0;
+  T e_kin = (sq(p.vel[0]) + sq(p.vel[1]) + sq(p.vel[2])) / 2.0;
   p.e_int = p.e_total - e_kin;
   for (int k = 0; k < 9; ++k) p.def_grad[k] = Q[6 + k] / p.den;
 }
 
-inline void cons2prim_phase(const Barton2009& e, const double* Q, double* P) {  // :106-133
-  PhasePrims p;
+template <class T, class E>
+inline void cons2prim_phase(const E& e, const T* Q, T* P) {  // :106-133
+  PhasePrimsT<T> p;
   prims_phase(e, Q, p);
-  double G[9];
+  T G[9];
   finger(p.def_grad, G);
-  double ent = entropy(e, p.e_int, G);
+  T ent = entropy(e, p.e_int, G);
   P[0] = p.frac;
   P[1] = p.true_den;
   for (int k = 0; k < 3; ++k) P[2 + k] = p.vel[k];
@@ -451,38 +524,40 @@ inline void cons2prim_phase(const Barton2009& e, const double* Q, double* P) {  
   for (int k = 0; k < 9; ++k) P[6 + k] = p.def_grad[k];
 }
 
-inline void flux_phase(const Barton2009& e, const double* Q, double* F) {  // :146-175
-  PhasePrims p;
+template <class T, class E>
+inline void flux_phase(const E& e, const T* Q, T* F) {  // :146-175
+  PhasePrimsT<T> p;
   prims_phase(e, Q, p);
-  double G[9], sig[9], strs[9];
+  T G[9], sig[9], strs[9];
   finger(p.def_grad, G);
-  double ent = entropy(e, p.e_int, G);
-  stress_ent<double>(e, ent, p.def_grad, sig);
+  T ent = entropy(e, p.e_int, G);
+  stress_ent<T>(e, ent, p.def_grad, sig);
   for (int k = 0; k < 9; ++k) strs[k] = p.frac * sig[k];
-  const double* v = p.vel;
-  const double* A = p.def_grad;
-  F[0] = 0.0;
+  const T* v = p.vel;
+  const T* A = p.def_grad;
+  F[0] = T(0.0);
   F[1] = p.den * v[0];
   for (int k = 0; k < 3; ++k) F[2 + k] = p.den * v[0] * v[k] - strs[3 * k];  // strs[1:3:end]
   F[5] = p.den * v[0] * p.e_total - (v[0] * strs[0] + v[1] * strs[3] + v[2] * strs[6]);
   // den .* (vel[1] .* A - vec(vel * transpose(A[1:3:end])))
   for (int c = 0; c < 3; ++c)
     for (int r = 0; r < 3; ++r) {
-      double outer = v[r] * A[3 * c];  // vel[r] * A(1,c)
+      T outer = v[r] * A[3 * c];  // vel[r] * A(1,c)
       F[6 + r + 3 * c] = p.den * (v[0] * A[r + 3 * c] - outer);
     }
 }
 
 // get_eigvals(eos::T, Q, n) :258-266 -> [u.n + c_k (k=1..3), u.n - c_k]
-inline void eigvals_phase(const Barton2009& e, const double* Q, const double* n, double* out) {
-  double P[15];
+template <class T, class E>
+inline void eigvals_phase(const E& e, const T* Q, const double* n, T* out) {
+  T P[15];
   cons2prim_phase(e, Q, P);
-  double ac[9], w[3];
+  T ac[9], w[3];
   acoustic_ent(e, P[5], P + 6, n, ac);
   eigvals3(ac, w);
-  double spd = P[2] * n[0] + P[3] * n[1] + P[4] * n[2];
+  T spd = P[2] * n[0] + P[3] * n[1] + P[4] * n[2];
   for (int k = 0; k < 3; ++k) {
-    double c = std::sqrt(std::fabs(w[k]));
+    T c = sqrt_(abs_(w[k]));
     out[k] = spd + c;
     out[3 + k] = spd - c;
   }
@@ -566,10 +641,12 @@ static const double GLO6_W[6] = {0.066666666666666666667, 0.37847495629784698032
                                  0.55485837703548635301,  0.55485837703548635301,
                                  0.37847495629784698032,  0.066666666666666666667};
 
-struct Model {
+template <class P>
+struct ModelT {
   int nph;             // 1 (14-component single phase) or 2 (30 components)
-  Barton2009 eos[2];
+  BartonT<P> eos[2];
 };
+using Model = ModelT<double>;
 
 // Bint over the straight path Q_a -> Q_b with the given rule (two phases only)
 inline void B_int2(const Model& m, const double* Qa, const double* Qb, const double* X,
@@ -596,17 +673,20 @@ inline void B_int2(const Model& m, const double* Qa, const double* Qb, const dou
 
 // Phase-block helpers on the model's state layout.  The single-phase state is
 // embedded into a 15-component block with a == 1 for the per-phase functions.
-inline int ncomp(const Model& m) { return m.nph == 1 ? 14 : 30; }
+template <class P>
+inline int ncomp(const ModelT<P>& m) { return m.nph == 1 ? 14 : 30; }
 
-inline void to_blocks(const Model& m, const double* Q, double* Qb) {
+template <class T, class P>
+inline void to_blocks(const ModelT<P>& m, const T* Q, T* Qb) {
   if (m.nph == 1) {
-    Qb[0] = 1.0;
+    Qb[0] = T(1.0);
     for (int k = 0; k < 14; ++k) Qb[1 + k] = Q[k];
   } else {
     for (int k = 0; k < 30; ++k) Qb[k] = Q[k];
   }
 }
-inline void from_blocks(const Model& m, const double* Qb, double* Q) {
+template <class T, class P>
+inline void from_blocks(const ModelT<P>& m, const T* Qb, T* Q) {
   if (m.nph == 1) {
     for (int k = 0; k < 14; ++k) Q[k] = Qb[1 + k];
   } else {
@@ -614,31 +694,38 @@ inline void from_blocks(const Model& m, const double* Qb, double* Q) {
   }
 }
 
-inline void flux_mph(const Model& m, const double* Qb, double* Fb) {  // :140-144
+template <class T, class P>
+inline void flux_mph(const ModelT<P>& m, const T* Qb, T* Fb) {  // :140-144
   for (int q = 0; q < m.nph; ++q) flux_phase(m.eos[q], Qb + 15 * q, Fb + 15 * q);
 }
 
-inline void get_eigvals(const Model& m, const double* Qb, double* out) {  // :252-256
+template <class T, class P>
+inline void get_eigvals(const ModelT<P>& m, const T* Qb, T* out) {  // :252-256
   const double n[3] = {1.0, 0.0, 0.0};
   for (int q = 0; q < m.nph; ++q) eigvals_phase(m.eos[q], Qb + 15 * q, n, out + 6 * q);
 }
 
 // lxf(eos, Q_l, Q_r, lambda) NumFluxes.jl:25-60 ; block layout in/out
-inline void lxf(const Model& m, const double* Ql, const double* Qr, double lambda, double* cons,
-                double* dm, double* dp) {
+// (the two-phase path integral in double precision only: the time-loop duals
+// below run the single-phase model)
+template <class T, class P>
+inline void lxf(const ModelT<P>& m, const T* Ql, const T* Qr, const T& lambda, T* cons, T* dm,
+                T* dp) {
   int nb = 15 * m.nph;
-  double Fl[30], Fr[30];
+  T Fl[30], Fr[30];
   flux_mph(m, Ql, Fl);
   flux_mph(m, Qr, Fr);
   for (int r = 0; r < nb; ++r) cons[r] = 0.5 * (Fl[r] + Fr[r]) - 0.5 * lambda * (Qr[r] - Ql[r]);
-  if (m.nph == 2) {
-    double fp[30];
-    B_int2(m, Ql, Qr, GLO6_X, GLO6_W, fp);
-    for (int r = 0; r < nb; ++r) { dp[r] = (1.0 / 2.0) * fp[r]; dm[r] = (1.0 / 2.0) * fp[r]; }
-  } else {
-    // one phase: the path integral multiplies the volume-fraction jump, which is 0
-    for (int r = 0; r < nb; ++r) { dp[r] = 0.0; dm[r] = 0.0; }
+  if constexpr (std::is_same<T, double>::value) {
+    if (m.nph == 2) {
+      double fp[30];
+      B_int2(m, Ql, Qr, GLO6_X, GLO6_W, fp);
+      for (int r = 0; r < nb; ++r) { dp[r] = (1.0 / 2.0) * fp[r]; dm[r] = (1.0 / 2.0) * fp[r]; }
+      return;
+    }
   }
+  // one phase: the path integral multiplies the volume-fraction jump, which is 0
+  for (int r = 0; r < nb; ++r) { dp[r] = T(0.0); dm[r] = T(0.0); }
 }
 
 // hll(eos, Q_l, Q_r, eigvals) NumFluxes.jl:70-132 ; eig_l/eig_r have 6*nph entries
@@ -683,12 +770,22 @@ inline void hll(const Model& m, const double* Ql, const double* Qr, const double
 
 enum { FLUX_RUSANOV = 0, FLUX_HLL = 1, FLUX_LXF = 2 };
 
+// max as Base.max / std::max order it: b if a < b, else a (by value)
+template <class T> inline T max_(const T& a, const T& b) { return val(a) < val(b) ? b : a; }
+
 // main.jl:202-241 time loop; Q is ncomp x nx column-major (component fastest).
 // Stops after max_steps steps or once t >= t_end (checked before each step).
-inline int run(const Model& m, int flux, int nx, double* Q, double cfl, double dx, int max_steps,
+// T: double, or a first-order dual carrying tangents of the initial state and
+// of the material constants through every step (Rusanov and Lax-Friedrichs,
+// single phase; the dual's time t and dt_hist record values).
+template <class T, class P>
+inline int run(const ModelT<P>& m, int flux, int nx, T* Q, double cfl, double dx, int max_steps,
                double t_end, double* t_io, int* steps_out, double* dt_hist, int nthreads) {
   const int nb = 15 * m.nph, nc = ncomp(m), ne = 6 * m.nph;
-  std::vector<double> Q0((size_t)nb * nx), Q1((size_t)nb * nx), eig((size_t)ne * nx), lam(nx);
+  if constexpr (!std::is_same<T, double>::value) {
+    if (m.nph != 1 || flux == FLUX_HLL) return -1;
+  }
+  std::vector<T> Q0((size_t)nb * nx), Q1((size_t)nb * nx), eig((size_t)ne * nx), lam(nx);
   for (int i = 0; i < nx; ++i) to_blocks(m, Q + (size_t)nc * i, Q0.data() + (size_t)nb * i);
   double t = *t_io;
   int steps = 0;
@@ -700,41 +797,46 @@ inline int run(const Model& m, int flux, int nx, double* Q, double cfl, double d
   while (steps < max_steps && t < t_end) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i) {
-      double* ev = eig.data() + (size_t)ne * i;
+      T* ev = eig.data() + (size_t)ne * i;
       get_eigvals(m, Q0.data() + (size_t)nb * i, ev);
-      double mx = 0.0;
-      for (int k = 0; k < ne; ++k) mx = std::max(mx, std::fabs(ev[k]));
+      T mx = T(0.0);
+      for (int k = 0; k < ne; ++k) mx = max_(mx, abs_(ev[k]));
       lam[i] = mx;
     }
-    double lmax = lam[0];
-    for (int i = 1; i < nx; ++i) lmax = std::max(lmax, lam[i]);
-    double dt = cfl * dx / lmax;
-    if (dt_hist) dt_hist[steps] = dt;
-    t += dt;
+    T lmax = lam[0];
+    for (int i = 1; i < nx; ++i) lmax = max_(lmax, lam[i]);
+    T dt = cfl * dx / lmax;
+    if (dt_hist) dt_hist[steps] = val(dt);
+    t += val(dt);
     steps += 1;
     for (int r = 0; r < nb; ++r) {
       Q1[r] = Q0[r];
       Q1[(size_t)nb * (nx - 1) + r] = Q0[(size_t)nb * (nx - 1) + r];
     }
-    double dtdx = dt / dx;
+    T dtdx = dt / dx;
 #pragma omp parallel for schedule(static)
     for (int i = 1; i < nx - 1; ++i) {
-      const double* Ql = Q0.data() + (size_t)nb * (i - 1);
-      const double* Qc = Q0.data() + (size_t)nb * i;
-      const double* Qr = Q0.data() + (size_t)nb * (i + 1);
-      double Fl[30], dml[30], NFl[30], Fr[30], NFr[30], dpr[30];
-      if (flux == FLUX_HLL) {  // main.jl:56-59
-        hll(m, Ql, Qc, eig.data() + (size_t)ne * (i - 1), eig.data() + (size_t)ne * i, Fl, dml, NFl);
-        hll(m, Qc, Qr, eig.data() + (size_t)ne * i, eig.data() + (size_t)ne * (i + 1), Fr, NFr, dpr);
-      } else {  // Rusanov, main.jl:46-53 (commented variants)
+      const T* Ql = Q0.data() + (size_t)nb * (i - 1);
+      const T* Qc = Q0.data() + (size_t)nb * i;
+      const T* Qr = Q0.data() + (size_t)nb * (i + 1);
+      T Fl[30], dml[30], NFl[30], Fr[30], NFr[30], dpr[30];
+      if constexpr (std::is_same<T, double>::value) {
+        if (flux == FLUX_HLL) {  // main.jl:56-59
+          hll(m, Ql, Qc, eig.data() + (size_t)ne * (i - 1), eig.data() + (size_t)ne * i, Fl, dml,
+              NFl);
+          hll(m, Qc, Qr, eig.data() + (size_t)ne * i, eig.data() + (size_t)ne * (i + 1), Fr, NFr,
+              dpr);
+        }
+      }
+      if (flux != FLUX_HLL) {  // Rusanov, main.jl:46-53 (commented variants)
         // Rusanov: lambda_l, lambda_r = max over the face's two cells (main.jl:47-48);
         // Lax-Friedrichs: lambda_l, lambda_r = 1 / dtdx, 1 / dtdx (main.jl:50)
-        double lam_l = flux == FLUX_LXF ? 1 / dtdx : std::max(lam[i - 1], lam[i]);
-        double lam_r = flux == FLUX_LXF ? 1 / dtdx : std::max(lam[i], lam[i + 1]);
+        T lam_l = flux == FLUX_LXF ? 1 / dtdx : max_(lam[i - 1], lam[i]);
+        T lam_r = flux == FLUX_LXF ? 1 / dtdx : max_(lam[i], lam[i + 1]);
         lxf(m, Ql, Qc, lam_l, Fl, dml, NFl);
         lxf(m, Qc, Qr, lam_r, Fr, NFr, dpr);
       }
-      double* out = Q1.data() + (size_t)nb * i;
+      T* out = Q1.data() + (size_t)nb * i;
       for (int r = 0; r < nb; ++r) out[r] = Qc[r] - dtdx * ((Fr[r] - Fl[r]) + (NFr[r] + NFl[r]));
     }
     std::swap(Q0, Q1);
@@ -787,6 +889,19 @@ void ora_acoustic_prim(const double* mat, const double* P, const double* n, doub
   acoustic_prim(make_barton(mat), P, n, ac);
 }
 void ora_eigvals3(const double* a, double* w) { eigvals3(a, w); }
+// eigenvalues of a + s da and their derivatives in s at s = 0 (first-order duals)
+void ora_eigvals3_dual(const double* a, const double* da, double* w, double* dw) {
+  Dual<double, 1> ad[9], wd[3];
+  for (int k = 0; k < 9; ++k) {
+    ad[k] = Dual<double, 1>(a[k]);
+    ad[k].d[0] = da[k];
+  }
+  eigvals3(ad, wd);
+  for (int i = 0; i < 3; ++i) {
+    w[i] = wd[i].v;
+    dw[i] = wd[i].d[0];
+  }
+}
 double ora_temperature(const double* mat, double S, const double* G) {
   typedef Dual<double, 1> D1;
   D1 Sd(S);
@@ -866,6 +981,72 @@ int ora_run(int nph, const double* mats, int flux, int nx, double* Q, double cfl
             int nthreads) {
   Model m = make_model(nph, mats);
   return run(m, flux, nx, Q, cfl, dx, max_steps, t_end, t_io, steps_out, dt_hist, nthreads);
+}
+
+// Loss of the single-phase Rusanov run from two pieces and its gradient by
+// forward-mode duals through the whole time loop (ForwardDiff.gradient of the
+// misfit, restated): 28 tangents, the 13 primitives u(3), vec(A), S of each
+// piece (rho = rho0 / det A) and the constants b0 (shear) and c0 (bulk).
+// prims: 13 x 2 column-major; cells i with (i + 1/2) / nx > x_ifc take piece 2.
+// loss = sum_i (rho_i - rho_t_i)^2 + (u1_i - u_t_i)^2 with rho, u1 of cons2prim.
+// Returns the number of steps run, or -1.
+int ora_adjoint_dual(const double* mats, const double* prims, double x_ifc, int nx, int steps,
+                     double cfl, const double* rho_t, const double* u_t, int nthreads,
+                     double* loss, double* grad) {
+  typedef Dual<double, 28> D;
+  ModelT<D> m;
+  m.nph = 1;
+  BartonT<D>& e = m.eos[0];
+  e.rho0 = mats[0];
+  e.c0 = D(mats[1]);
+  e.c0.d[27] = 1.0;
+  e.cv = mats[2];
+  e.t0 = mats[3];
+  e.b0 = D(mats[4]);
+  e.b0.d[26] = 1.0;
+  e.alpha = mats[5];
+  e.beta = mats[6];
+  e.gamma = mats[7];
+  e.b0sq = e.b0 * e.b0;                                // EquationsOfState.jl:90-115
+  e.k0 = e.c0 * e.c0 - (4.0 / 3.0) * (e.b0 * e.b0);
+  m.eos[1] = e;
+  D Qs[2][14];
+  for (int j = 0; j < 2; ++j) {
+    D x[13];
+    for (int k = 0; k < 13; ++k) {
+      x[k] = D(prims[13 * j + k]);
+      x[k].d[13 * j + k] = 1.0;
+    }
+    D P[15], Q[15];
+    P[0] = D(1.0);
+    P[1] = e.rho0 / det_(x + 3);
+    for (int k = 0; k < 3; ++k) P[2 + k] = x[k];
+    P[5] = x[12];
+    for (int k = 0; k < 9; ++k) P[6 + k] = x[3 + k];
+    prim2cons_phase(e, P, Q);
+    for (int k = 0; k < 14; ++k) Qs[j][k] = Q[1 + k];
+  }
+  std::vector<D> Q((size_t)14 * nx);
+  for (int i = 0; i < nx; ++i) {
+    const int j = (i + 0.5) / nx > x_ifc ? 1 : 0;
+    for (int k = 0; k < 14; ++k) Q[(size_t)14 * i + k] = Qs[j][k];
+  }
+  double t = 0.0;
+  int done = 0;
+  if (run(m, FLUX_RUSANOV, nx, Q.data(), cfl, 1.0 / nx, steps, INFINITY, &t, &done, nullptr,
+          nthreads) != 0)
+    return -1;
+  D L = D(0.0);
+  for (int i = 0; i < nx; ++i) {
+    D Qb[30];
+    to_blocks(m, Q.data() + (size_t)14 * i, Qb);
+    PhasePrimsT<D> p;
+    prims_phase(e, Qb, p);
+    L = L + (sq(p.true_den - rho_t[i]) + sq(p.vel[0] - u_t[i]));
+  }
+  *loss = L.v;
+  for (int k = 0; k < 28; ++k) grad[k] = L.d[k];
+  return done;
 }
 
 }  // extern "C"

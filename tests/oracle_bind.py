@@ -156,6 +156,24 @@ class Oracle:
         return Q, t.value, s.value, hist[: s.value].copy()
 
 
+    def adjoint_dual(self, mats, prims, x_ifc, nx, steps, cfl, rho_t, u_t, threads=0):
+        """Misfit of the single-phase Rusanov run from two pieces (prims: 13 x 2,
+        u(3), vec(A), S) and its gradient with respect to both pieces' prims and
+        (b0, c0), by forward-mode duals through the whole time loop."""
+        pr = np.asfortranarray(np.array(prims, dtype=float))
+        rt = np.ascontiguousarray(np.array(rho_t, dtype=float))
+        ut = np.ascontiguousarray(np.array(u_t, dtype=float))
+        assert pr.shape == (13, 2) and rt.size == nx and ut.size == nx
+        loss = ctypes.c_double()
+        g = np.zeros(28)
+        done = self.L.ora_adjoint_dual(P(np.ascontiguousarray(mats, dtype=float)), P(pr), _d(x_ifc),
+                                       int(nx), int(steps), _d(cfl), P(rt), P(ut), int(threads),
+                                       ctypes.byref(loss), P(g))
+        if done != steps:
+            raise RuntimeError(f"dual run stopped after {done} of {steps} steps")
+        return loss.value, g
+
+
 _ORA = None
 
 

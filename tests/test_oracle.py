@@ -608,3 +608,93 @@ def test_oracle_reproduces_golden_fixtures(ora):
     g = np.load(os.path.join(here, "config1_rusanov.npz"))
     Q, t, s, dt = ora.run(1, g["mat"], 0, g["Q0"], 0.6, 1.0 / g["Q0"].shape[1], t_end=0.06)
     assert s == int(g["steps"]) and np.array_equal(Q, g["Q"])
+
+
+def _dual_case(ora, n, steps, seed=13):
+    """Pieces, constants and a target for the dual checks (the target: the
+    oracle's run with b0 x 1.05 and c0 x 0.97)."""
+    from golden.make_adjoint_dual import oracle_profiles, pieces
+    mats, prims = pieces(seed)
+    mt = mats.copy()
+    mt[4] *= 1.05
+    mt[1] *= 0.97
+    rho_t, u_t = oracle_profiles(ora, mt, prims, n, steps)
+    return mats, prims, rho_t, u_t
+
+
+def _theta_loss(ora, theta, n, steps, rho_t, u_t):
+    """The misfit as a function of the 28 scalars, through the double oracle."""
+    from golden.make_adjoint_dual import oracle_profiles
+    prims = np.asfortranarray(np.stack([theta[:13], theta[13:26]], axis=1))
+    mats = MAT.copy()
+    mats[4], mats[1] = theta[26], theta[27]
+    rho, u1 = oracle_profiles(ora, mats, prims, n, steps)
+    return float(np.sum((rho - rho_t) ** 2) + np.sum((u1 - u_t) ** 2))
+
+
+def test_dual_gradient_matches_central_differences(ora):
+    """The 28 partials that the duals carry through the time loop (64 cells,
+    10 Rusanov steps) against central differences of the double oracle,
+    h = 1e-6 |theta_k|.  The dual's value is the double run's misfit."""
+    n, steps = 64, 10
+    mats, prims, rho_t, u_t = _dual_case(ora, n, steps)
+    # a target away from the run itself, so that every partial is sizeable
+    rho_t = rho_t * (1.0 + 0.01 * np.sin(np.arange(n)))
+    L, g = ora.adjoint_dual(mats, prims, 0.5, n, steps, 0.6, rho_t, u_t)
+    theta = np.concatenate([prims[:, 0], prims[:, 1], [mats[4], mats[1]]])
+    assert L == pytest.approx(_theta_loss(ora, theta, n, steps, rho_t, u_t), rel=1e-12)
+    fd = np.empty(28)
+    for k in range(28):
+        h = 1e-6 * abs(theta[k])
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd[k] = (_theta_loss(ora, tp, n, steps, rho_t, u_t) -
+                 _theta_loss(ora, tm, n, steps, rho_t, u_t)) / (2 * h)
+    # the differences' rounding error is about eps L / h: 2e-8 of the largest
+    # partial for the off-diagonal entries of A (|theta| = 0.05)
+    scale = np.abs(g).max()
+    for k in range(28):
+        assert abs(g[k] - fd[k]) <= 5e-6 * abs(g[k]) + 2e-8 * scale, (k, g[k], fd[k])
+
+
+def test_dual_eigenvalue_partials_match_differences(ora):
+    """The dual eigenvalues' partials (v^T dA v with v from the Jacobi
+    rotations) against central differences of the Jacobi eigenvalues, on
+    acoustic tensors of random states moved along random directions."""
+    import ctypes
+    from oracle_bind import P as ptr
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        A = rand_A(rng)
+        ac = ora.acoustic(MAT, 1e-3, A.reshape(-1, order="F")).reshape(-1, order="F")
+        d = rng.standard_normal((3, 3))
+        d = (d + d.T).reshape(-1, order="F")
+        w, dw = np.zeros(3), np.zeros(3)
+        ora.L.ora_eigvals3_dual(ptr(ac), ptr(d), ptr(w), ptr(dw))
+        assert np.array_equal(w, ora.eigvals3(ac))
+        h = 1e-6 * np.abs(ac).max()
+        fd = (ora.eigvals3(ac + h * d) - ora.eigvals3(ac - h * d)) / (2 * h)
+        # the Jacobi order of the eigenvalues is the same at +-h (they are well apart)
+        np.testing.assert_allclose(dw, fd, rtol=1e-6, atol=1e-8 * np.abs(d).max())
+        # first-order perturbation of a symmetric matrix's eigenvalues, from
+        # LAPACK's eigenvectors (eigh: ascending eigenvalues)
+        ev, V = np.linalg.eigh(0.5 * (ac + ac.reshape(3, 3).reshape(-1, order="F")).reshape(3, 3, order="F"))
+        Dm = d.reshape(3, 3, order="F")
+        ref = np.array([V[:, i] @ Dm @ V[:, i] for i in range(3)])
+        np.testing.assert_allclose(dw[np.argsort(w)], ref, rtol=1e-9, atol=1e-12 * np.abs(d).max())
+
+
+@pytest.mark.slow
+def test_adjoint_dual_fixture_reproduces(ora):
+    """tests/golden/adjoint_dual_512.npz (512 cells, 120 steps; written by
+    tests/golden/make_adjoint_dual.py) is what the duals give now: the GPU
+    adjoint's gradient is checked against it."""
+    import os
+    from golden.make_adjoint_dual import FIXTURE, make
+    g = np.load(FIXTURE)
+    d = make(ora)
+    assert np.array_equal(d["rho_t"], g["rho_t"]) and np.array_equal(d["u_t"], g["u_t"])
+    assert d["loss"] == float(g["loss"])
+    assert np.array_equal(d["grad"], g["grad"])
+    assert os.path.basename(FIXTURE) == "adjoint_dual_512.npz"
