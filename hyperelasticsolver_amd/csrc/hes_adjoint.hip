@@ -121,17 +121,27 @@ struct AdjFwdArgs {
   const double* __restrict__ Cin;
   double* __restrict__ Cout;
   unsigned long long* lmax;  // lmax[s] = max CFL speed of state s (double bits)
-  double* blk_val;           // optional: per-block argmax of the CFL speed of state s + 1
+  double* blk_val;           // FWD_RECORD: per-block argmax of the CFL speed of state s + 1
   long long* blk_idx;
+  long long* istar;          // FWD_FIND: argmax cell of every state
   int* status;
   long long n, pitch, s;
   double cfl, dx;
-  int write_lmax;            // max-reduce the new state's speeds into lmax[s + 1]
   MatDev mat;
 };
 
+// What a forward step records of the new state s + 1's CFL speeds:
+//   FWD_MAX     the first pass: their max, into lmax[s + 1]
+//   FWD_RECORD  the first pass at a checkpoint state: also the per-block argmax
+//   FWD_FIND    a recompute of the reverse pass: the argmax cell (smallest
+//               index) of the max that the first pass recorded, into istar[s + 1]
+// The argmax of the states between checkpoints is found while the reverse
+// pass recomputes them, so the first pass does no more than a plain run's max.
+enum { FWD_MAX = 0, FWD_RECORD = 1, FWD_FIND = 2 };
+
 // One time step: the solver's chunk step (so states match a plain run bit for
 // bit) plus the bookkeeping of the reverse pass.
+template <int MODE>
 __global__ __launch_bounds__(ABLK) __attribute__((amdgpu_waves_per_eu(3))) void k_adj_fwd(
     AdjFwdArgs a) {
   const int lane = threadIdx.x & 63;
@@ -142,20 +152,39 @@ __global__ __launch_bounds__(ABLK) __attribute__((amdgpu_waves_per_eu(3))) void 
   load_chunk<1>(ctx, i, q, cc);
   const double lmax = lmax_of(a.lmax, a.s);
   const double lam = chunk_step<1, FLUX_RUSANOV, 1>(ctx, i, q, cc, lmax);
-  if (a.write_lmax) {
-    const bool owned = i >= 0 && i < a.n && lane >= 1 && lane <= TILE;
+  const bool owned = i >= 0 && i < a.n && lane >= 1 && lane <= TILE;
+  if constexpr (MODE == FWD_FIND) {
+    // the recomputed speeds are those of the first pass, bit for bit
+    if (owned && lam == lmax_of(a.lmax, a.s + 1))
+      atomicMin((unsigned long long*)(a.istar + a.s + 1), (unsigned long long)i);
+  } else if constexpr (MODE == FWD_RECORD) {
     const long long row = (a.s + 1) * (long long)gridDim.x;
-    record_speeds<ABLK>(lam, owned, i, a.lmax + a.s + 1, a.blk_val ? a.blk_val + row : nullptr,
-                        a.blk_idx ? a.blk_idx + row : nullptr);
+    record_speeds<ABLK>(lam, owned, i, a.lmax + a.s + 1, a.blk_val + row, a.blk_idx + row);
+  } else {
+    // one atomic per wave (most find a larger maximum already there)
+    const double m = wave_max(owned ? lam : -1.0);
+    if (lane == 0 && m > 0.0) {
+      unsigned long long* slot = a.lmax + a.s + 1;
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+      if (bits > __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(slot, bits);
+    }
   }
 }
 
-// Argmax cell of the CFL speed of every state from the per-block records.
+// Argmax cell of the CFL speed of the checkpoint states from the per-block
+// records (state 0: the per-cell speeds kernel's); the other states' entries
+// are set to NO_CELL, for the reverse pass's recomputes to min-reduce into.
 __global__ __launch_bounds__(64) void k_adj_argmax(const double* __restrict__ blk_val,
                                                    const long long* __restrict__ blk_idx, int nb,
-                                                   int nb0, long long* __restrict__ istar) {
+                                                   int nb0, long long every,
+                                                   long long* __restrict__ istar) {
   const long long s = blockIdx.x;
   const int lane = threadIdx.x;
+  if (s % every != 0) {
+    if (lane == 0) istar[s] = NO_CELL;
+    return;
+  }
   const int cnt = s == 0 ? nb0 : nb;  // state 0 was recorded by the per-cell speeds kernel
   double v = -2.0;
   long long i = NO_CELL;
@@ -547,24 +576,22 @@ double* seg_slot(hes_adjoint* h, long long s, long long s0, int buf) {
 }
 
 AdjFwdArgs fwd_args(hes_adjoint* h, long long s, const double* qi, double* qo, const double* ci,
-                    double* co, bool first_pass) {
+                    double* co) {
   AdjFwdArgs a;
   a.Qin = qi;
   a.Qout = qo;
   a.Cin = ci;
   a.Cout = co;
   a.lmax = h->lmax;
-  // rows 1 .. nsteps-1 of the argmax record (the final state needs none)
-  const bool rec = first_pass && s + 1 < h->nsteps;
-  a.blk_val = rec ? h->blk_val : nullptr;
-  a.blk_idx = rec ? h->blk_idx : nullptr;
+  a.blk_val = h->blk_val;
+  a.blk_idx = h->blk_idx;
+  a.istar = h->istar;
   a.status = h->status;
   a.n = h->n;
   a.pitch = h->pitch;
   a.s = s;
   a.cfl = h->cfl;
   a.dx = h->dx;
-  a.write_lmax = first_pass ? 1 : 0;
   a.mat = h->mat;
   return a;
 }
@@ -587,11 +614,17 @@ hipError_t record_forward(hes_adjoint* h) {
     double *qi, *ci, *qo, *co;
     fwd_slot(h, s, &qi, &ci);
     fwd_slot(h, s + 1, &qo, &co);
-    hipLaunchKernelGGL(k_adj_fwd, dim3((unsigned)h->nb), dim3(ABLK), 0, st,
-                       fwd_args(h, s, qi, qo, ci, co, true));
+    // per-block argmax records of the checkpoint states 1 .. nsteps-1 (the
+    // final state needs none)
+    if ((s + 1) % h->every == 0 && s + 1 < h->nsteps)
+      hipLaunchKernelGGL(k_adj_fwd<FWD_RECORD>, dim3((unsigned)h->nb), dim3(ABLK), 0, st,
+                         fwd_args(h, s, qi, qo, ci, co));
+    else
+      hipLaunchKernelGGL(k_adj_fwd<FWD_MAX>, dim3((unsigned)h->nb), dim3(ABLK), 0, st,
+                         fwd_args(h, s, qi, qo, ci, co));
   }
   hipLaunchKernelGGL(k_adj_argmax, dim3((unsigned)h->nsteps), dim3(64), 0, st, h->blk_val,
-                     h->blk_idx, h->nb, h->nbl, h->istar);
+                     h->blk_idx, h->nb, h->nbl, h->every, h->istar);
   double *qn, *cn;
   fwd_slot(h, h->nsteps, &qn, &cn);
   hipLaunchKernelGGL(k_adj_loss, dim3(nbl), dim3(LBLK), 0, st, qn, h->rho_t, h->u_t, h->mat8[0],
@@ -636,9 +669,9 @@ hipError_t record_backward(hes_adjoint* h) {
       // as the checkpoints do: the reverse step reads its face speeds there
       const double* ci = seg_slot(h, s, s0, buf) + 14 * h->pitch;
       double* co = seg_slot(h, s + 1, s0, buf) + 14 * h->pitch;
-      hipLaunchKernelGGL(k_adj_fwd, dim3((unsigned)h->nb), dim3(ABLK), 0, st2,
+      hipLaunchKernelGGL(k_adj_fwd<FWD_FIND>, dim3((unsigned)h->nb), dim3(ABLK), 0, st2,
                          fwd_args(h, s, seg_slot(h, s, s0, buf), seg_slot(h, s + 1, s0, buf), ci,
-                                  co, false));
+                                  co));
     }
     const int cnt = (int)(end - s0);
     hipLaunchKernelGGL(k_adj_lam, dim3((cnt + 63) / 64), dim3(64), 0, st2, ck_q(h, c),
