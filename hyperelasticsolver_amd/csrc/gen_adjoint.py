@@ -185,54 +185,31 @@ def flux(m, t):
 
 
 def acoustic_e1(m, t):
+    """O11 O12 O13 O22 O23 O33 for n = e1 in the collected form the forward
+    step uses (alpha = 1, beta = 3, gamma = 2): with T = G^2 - (I1/3) G and
+    2 kappa = b0^2 I3^(3/2),
+      O11 = 2 kappa (9 T11 + 4/3 G11^2) + s_a k0 (3 s_a - 2) + 6 E_th + 12 W,
+      O1k = 2 kappa (5 T1k + 4/3 G11 G1k),  Ojk = 2 kappa (Tjk + G11 Gjk + 1/3 G1j G1k)."""
     G11, G22, G33, G12, G13, G23 = (t[k] for k in ("G11", "G22", "G33", "G12", "G13", "G23"))
-    G2_11 = G11 * G11 + G12 * G12 + G13 * G13
-    G2_12 = G11 * G12 + G12 * G22 + G13 * G23
-    G2_13 = G11 * G13 + G12 * G23 + G13 * G33
-    G2_22 = G12 * G12 + G22 * G22 + G23 * G23
-    G2_23 = G12 * G13 + G22 * G23 + G23 * G33
-    G2_33 = G13 * G13 + G23 * G23 + G33 * G33
-    kap = m["hb0sq"] * t["sb"]
-    c3 = c3_of(m, t)
-    i1 = t["i1"]
-    i13 = i1 * (1.0 / 3.0)
-    H11 = kap * (G2_11 - i13 * G11) + c3
-    H12 = kap * (G2_12 - i13 * G12)
-    H13 = kap * (G2_13 - i13 * G13)
-    hs = m["hb0sq"] * t["sb"]
-    beta, gamma = 3.0, 2.0
-    # direction j = 1
-    di1 = -2.0 * G11
-    di2 = i1 * di1 + 2.0 * G2_11
-    dkap = -beta * kap
-    dW = -beta * t["W"] + hs * ((2.0 / 3.0) * i1 * di1 - di2)
-    dc3 = -m["k0_2"] * (2.0 * t["sa"] - 1.0) * t["sa"] - (0.5 * gamma * gamma) * t["Eth"] + 1.5 * dW
-    di13 = di1 * (1.0 / 3.0)
-    dG = -2.0 * G11
-    dG2 = -2.0 * (G2_11 + G11 * G11)
-    dH = dkap * (G2_11 - i13 * G11) + kap * (dG2 - di13 * G11 - i13 * dG) + dc3
-    O11 = 2.0 * H11 - 2.0 * dH
-    dG = -G12
-    dG2 = -(2.0 * G11 * G12 + G2_12)
-    dH = dkap * (G2_12 - i13 * G12) + kap * (dG2 - di13 * G12 - i13 * dG)
-    O12 = 2.0 * H12 - 2.0 * dH
-    dG = -G13
-    dG2 = -(2.0 * G11 * G13 + G2_13)
-    dH = dkap * (G2_13 - i13 * G13) + kap * (dG2 - di13 * G13 - i13 * dG)
-    O13 = 2.0 * H13 - 2.0 * dH
-    # direction j = 2
-    di13 = -2.0 * G12 * (1.0 / 3.0)
-    dG = -G22
-    dG2 = -(G11 * G22 + G12 * G12 + G2_22)
-    O22 = -2.0 * (kap * (dG2 - di13 * G12 - i13 * dG))
-    dG = -G23
-    dG2 = -(G11 * G23 + G12 * G13 + G2_23)
-    O23 = -2.0 * (kap * (dG2 - di13 * G13 - i13 * dG))
-    # direction j = 3
-    di13 = -2.0 * G13 * (1.0 / 3.0)
-    dG = -G33
-    dG2 = -(G11 * G33 + G13 * G13 + G2_33)
-    O33 = -2.0 * (kap * (dG2 - di13 * G13 - i13 * dG))
+    i13 = t["i1"] * (1.0 / 3.0)
+    T11 = G11 * G11 + G12 * G12 + G13 * G13 - i13 * G11
+    T12 = G11 * G12 + G12 * G22 + G13 * G23 - i13 * G12
+    T13 = G11 * G13 + G12 * G23 + G13 * G33 - i13 * G13
+    T22 = G12 * G12 + G22 * G22 + G23 * G23 - i13 * G22
+    T23 = G12 * G13 + G22 * G23 + G23 * G33 - i13 * G23
+    T33 = G13 * G13 + G23 * G23 + G33 * G33 - i13 * G33
+    k2 = m["b0sq"] * t["sb"]
+    g43 = (4.0 / 3.0) * G11
+    sa = t["sa"]
+    O11 = k2 * (9.0 * T11 + g43 * G11) + (m["k0"] * sa * (3.0 * sa - 2.0) + 6.0 * t["Eth"] +
+                                          12.0 * t["W"])
+    O12 = k2 * (5.0 * T12 + g43 * G12)
+    O13 = k2 * (5.0 * T13 + g43 * G13)
+    g12 = (1.0 / 3.0) * G12
+    g13 = (1.0 / 3.0) * G13
+    O22 = k2 * (T22 + G11 * G22 + g12 * G12)
+    O23 = k2 * (T23 + G11 * G23 + g12 * G13)
+    O33 = k2 * (T33 + G11 * G33 + g13 * G13)
     return [O11, O12, O13, O22, O23, O33]
 
 

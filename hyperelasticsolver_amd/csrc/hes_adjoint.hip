@@ -45,7 +45,10 @@ using namespace hes_adj;
 
 namespace {
 
-constexpr int ABLK = 256;  // threads per block of the tiled kernels (4 waves)
+#ifndef HES_ADJ_BLOCK
+#define HES_ADJ_BLOCK 256
+#endif
+constexpr int ABLK = HES_ADJ_BLOCK;  // threads per block of the tiled kernels
 constexpr int LBLK = 256;  // threads per block of the one-cell-per-thread kernels
 
 // ---------------------------------------------------------------------------

@@ -31,12 +31,13 @@ struct MatDev {
   double rho0, irho0, cv, t0, b0sq, k0, alpha, beta, gamma;
   double hk0a2;   // k0 / (2 alpha^2)      : cold-energy prefactor
   double k0_2a;   // k0 / (2 alpha)        : in c3
-  double k0_2;    // k0 / 2                : in dc3
+  double k0_a;    // k0 / alpha            : in O11 of the acoustic tensor
   double cvt0;    // cv * t0
   double icvt0;   // 1 / (cv t0)
   double hb0sq;   // b0^2 / 2
   double half_a, half_b, half_g;  // alpha/2, beta/2, gamma/2
-  double g2_2;    // gamma^2 / 2
+  double ac_t, ac_s;  // 3 + 2 beta, 2 + beta  : acoustic tensor, shear-coupling terms
+  double ac_e, ac_w;  // gamma (1 + gamma), beta (1 + beta) : O11's E_th and W terms
   int fast;       // alpha == 1, beta == 3, gamma == 2 (no pow needed)
   int pad;
 };
@@ -241,66 +242,37 @@ __device__ __forceinline__ double temperature_of(const MatDev& m, const Thermo& 
 }
 
 // Acoustic tensor for n = e1 (the 1D solver's normal), symmetric: returns
-// O11 O12 O13 O22 O23 O33.
+// O11 O12 O13 O22 O23 O33.  The stress is -2 rho H with H = kappa T + c3 I,
+// T = G^2 - (I1/3) G and kappa = b0^2/2 I3^(beta/2) (stress_row1); the
+// acoustic tensor of EquationsOfState.jl:223-246 differentiates it along n,
+// with G' = -(e1 G_1. + G_.1 e1^T) and the invariants' and I3's chain rules.
+// Collecting the terms by hand gives
+//   O11 = 2 kappa ((3 + 2 beta) T11 + 4/3 G11^2)
+//         + s_a (k0/alpha (s_a - 1) + k0 (2 s_a - 1)) + gamma (1 + gamma) E_th
+//         + beta (1 + beta) W
+//   O1k = 2 kappa ((2 + beta) T1k + 4/3 G11 G1k)          k = 2, 3
+//   Ojk = 2 kappa (Tjk + G11 Gjk + 1/3 G1j G1k)           j, k = 2, 3
+// (the term-by-term derivative, which this replaces, agrees to round-off).
 __device__ __forceinline__ void acoustic_e1(const MatDev& m, const Thermo& t, double O[6]) {
   const double G11 = t.G11, G22 = t.G22, G33 = t.G33, G12 = t.G12, G13 = t.G13, G23 = t.G23;
-  double G2_11 = G11 * G11 + G12 * G12 + G13 * G13;
-  double G2_12 = G11 * G12 + G12 * G22 + G13 * G23;
-  double G2_13 = G11 * G13 + G12 * G23 + G13 * G33;
-  double G2_22 = G12 * G12 + G22 * G22 + G23 * G23;
-  double G2_23 = G12 * G13 + G22 * G23 + G23 * G33;
-  double G2_33 = G13 * G13 + G23 * G23 + G33 * G33;
-  double kap = m.hb0sq * t.sb;
-  double c3 = c3_of(m, t);
-  double i1 = t.i1, i13 = i1 * (1.0 / 3.0);
-  double H11 = kap * (G2_11 - i13 * G11) + c3;
-  double H12 = kap * (G2_12 - i13 * G12);
-  double H13 = kap * (G2_13 - i13 * G13);
-  double hs = m.hb0sq * t.sb;  // dW = -beta W n_j + hs ((2/3) i1 di1 - di2)
-  // --- direction j = 1 (n_j = 1)
-  {
-    double di1 = -2.0 * G11;
-    double di2 = i1 * di1 + 2.0 * G2_11;
-    double dkap = -m.beta * kap;
-    double dW = -m.beta * t.W + hs * ((2.0 / 3.0) * i1 * di1 - di2);
-    double dc3 = -m.k0_2 * (2.0 * t.sa - 1.0) * t.sa - m.g2_2 * t.Eth + m.half_b * dW;
-    double di13 = di1 * (1.0 / 3.0);
-    // i = 1
-    double dG = -2.0 * G11;
-    double dG2 = -2.0 * (G2_11 + G11 * G11);
-    double dH = dkap * (G2_11 - i13 * G11) + kap * (dG2 - di13 * G11 - i13 * dG) + dc3;
-    O[0] = 2.0 * H11 - 2.0 * dH;
-    // i = 2
-    dG = -G12;
-    dG2 = -(2.0 * G11 * G12 + G2_12);
-    dH = dkap * (G2_12 - i13 * G12) + kap * (dG2 - di13 * G12 - i13 * dG);
-    O[1] = 2.0 * H12 - 2.0 * dH;
-    // i = 3
-    dG = -G13;
-    dG2 = -(2.0 * G11 * G13 + G2_13);
-    dH = dkap * (G2_13 - i13 * G13) + kap * (dG2 - di13 * G13 - i13 * dG);
-    O[2] = 2.0 * H13 - 2.0 * dH;
-  }
-  // --- direction j = 2 (n_j = 0): i = 2, 3
-  {
-    double di13 = -2.0 * G12 * (1.0 / 3.0);
-    double dG = -G22;
-    double dG2 = -(G11 * G22 + G12 * G12 + G2_22);
-    double dH = kap * (dG2 - di13 * G12 - i13 * dG);
-    O[3] = -2.0 * dH;
-    dG = -G23;
-    dG2 = -(G11 * G23 + G12 * G13 + G2_23);
-    dH = kap * (dG2 - di13 * G13 - i13 * dG);
-    O[4] = -2.0 * dH;
-  }
-  // --- direction j = 3: i = 3
-  {
-    double di13 = -2.0 * G13 * (1.0 / 3.0);
-    double dG = -G33;
-    double dG2 = -(G11 * G33 + G13 * G13 + G2_33);
-    double dH = kap * (dG2 - di13 * G13 - i13 * dG);
-    O[5] = -2.0 * dH;
-  }
+  const double i13 = t.i1 * (1.0 / 3.0);
+  const double T11 = G11 * G11 + G12 * G12 + G13 * G13 - i13 * G11;
+  const double T12 = G11 * G12 + G12 * G22 + G13 * G23 - i13 * G12;
+  const double T13 = G11 * G13 + G12 * G23 + G13 * G33 - i13 * G13;
+  const double T22 = G12 * G12 + G22 * G22 + G23 * G23 - i13 * G22;
+  const double T23 = G12 * G13 + G22 * G23 + G23 * G33 - i13 * G23;
+  const double T33 = G13 * G13 + G23 * G23 + G33 * G33 - i13 * G33;
+  const double k2 = m.b0sq * t.sb;  // 2 kappa
+  const double g43 = (4.0 / 3.0) * G11;
+  const double sa = t.sa;
+  O[0] = k2 * (m.ac_t * T11 + g43 * G11) +
+         (sa * (m.k0_a * (sa - 1.0) + m.k0 * (2.0 * sa - 1.0)) + m.ac_e * t.Eth + m.ac_w * t.W);
+  O[1] = k2 * (m.ac_s * T12 + g43 * G12);
+  O[2] = k2 * (m.ac_s * T13 + g43 * G13);
+  const double g12 = (1.0 / 3.0) * G12, g13 = (1.0 / 3.0) * G13;
+  O[3] = k2 * (T22 + G11 * G22 + g12 * G12);
+  O[4] = k2 * (T23 + G11 * G23 + g12 * G13);
+  O[5] = k2 * (T33 + G11 * G33 + g13 * G13);
 }
 
 // Acoustic tensor for a general direction n, column-major 3x3.

@@ -44,14 +44,17 @@ inline MatDev make_mat(const double* p) {
   m.k0 = p[1] * p[1] - (4.0 / 3.0) * (b0 * b0);
   m.hk0a2 = 0.5 * m.k0 / (m.alpha * m.alpha);
   m.k0_2a = m.k0 / (2.0 * m.alpha);
-  m.k0_2 = 0.5 * m.k0;
+  m.k0_a = m.k0 / m.alpha;
   m.cvt0 = m.cv * m.t0;
   m.icvt0 = 1.0 / m.cvt0;
   m.hb0sq = 0.5 * m.b0sq;
   m.half_a = 0.5 * m.alpha;
   m.half_b = 0.5 * m.beta;
   m.half_g = 0.5 * m.gamma;
-  m.g2_2 = 0.5 * m.gamma * m.gamma;
+  m.ac_t = 3.0 + 2.0 * m.beta;
+  m.ac_s = 2.0 + m.beta;
+  m.ac_e = m.gamma * (1.0 + m.gamma);
+  m.ac_w = m.beta * (1.0 + m.beta);
   m.fast = (m.alpha == 1.0 && m.beta == 3.0 && m.gamma == 2.0) ? 1 : 0;
   return m;
 }
