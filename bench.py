@@ -436,7 +436,7 @@ def main():
                    "cells": nx, "flux": args.flux, "interfaces": 64},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(nx, args.flux),
-                     "kernel": f"k_step<1, {int(args.flux == 'hll')}, 1, {int(nx >= 1 << 21)}>",
+                     "kernel": f"k_step<1, {int(args.flux == 'hll')}, 1, {int(nx >= 1 << 21)}, 0>",
                      "bytes_per_cell_update": BYTES_PER_CELL_1PH,
                      "copy_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "device": device_info(),
