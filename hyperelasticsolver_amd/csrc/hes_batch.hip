@@ -450,11 +450,14 @@ void launch_batch(hes_batch* h, const BatchArgs& a) {
     const int tiles = a.tile_end - a.tile_begin;
     if (tiles <= 0) return;
     // jump faces first (a light scan), then their path terms over a fixed
-    // grid of at most 1024 waves, then the step
+    // grid (as many waves as stay resident: three per SIMD for Rusanov, one
+    // for HLL), then the step
     hipStream_t st = h->stream;
     hipLaunchKernelGGL(k_bmark2, dim3((unsigned)((tiles + MARK_BLK / 64 - 1) / (MARK_BLK / 64))),
                        dim3(MARK_BLK), 0, st, a);
-    const unsigned nbl = (unsigned)std::min<long long>(((long long)tiles * TILE + 254) / 256, 1024);
+    const long long nbf = ((long long)tiles * TILE + 254) / 256;
+    const unsigned nbl = (unsigned)std::min<long long>(
+        nbf, h->flux != HES_FLUX_HLL ? 3 * 1024 : 1024);
     if (h->flux != HES_FLUX_HLL) {
       if (h->fast) hipLaunchKernelGGL(k_bpath2<1>, dim3(nbl), dim3(64), 0, st, a);
       else hipLaunchKernelGGL(k_bpath2<0>, dim3(nbl), dim3(64), 0, st, a);

@@ -501,15 +501,18 @@ void launch_step_t(const StepArgs& a, long long nchunks, bool fast, hipStream_t 
   constexpr int WPB = BLOCK / 64;
   long long nblocks = (nchunks + WPB - 1) / WPB;
   if (NPH == 2) {
-    // jump faces first (a light scan), then their path terms over a fixed
-    // grid of at most one wave per SIMD
+    // jump faces first (a light scan), then their path terms over a fixed grid
     const unsigned nbp = (unsigned)((a.n + MARK_BLK - 1) / MARK_BLK);
     hipLaunchKernelGGL(k_mark2, dim3(nbp), dim3(MARK_BLK), 0, s, a);
-    const unsigned nbl = (unsigned)std::min<long long>((a.n + 254) / 256, 1024);  // 4 faces per block
+    // 4 faces per block; as many blocks as the kernels' occupancy keeps
+    // resident (1024 SIMDs): three waves per SIMD for Rusanov, one for HLL
+    const long long nbf = (a.n + 254) / 256;
     if (FLUX == FLUX_RUSANOV) {
+      const unsigned nbl = (unsigned)std::min<long long>(nbf, 3 * 1024);
       if (fast) hipLaunchKernelGGL(k_path2<1>, dim3(nbl), dim3(64), 0, s, a);
       else hipLaunchKernelGGL(k_path2<0>, dim3(nbl), dim3(64), 0, s, a);
     } else {
+      const unsigned nbl = (unsigned)std::min<long long>(nbf, 1024);
       if (fast) hipLaunchKernelGGL(k_path2_hll<1>, dim3(nbl), dim3(64), 0, s, a);
       else hipLaunchKernelGGL(k_path2_hll<0>, dim3(nbl), dim3(64), 0, s, a);
     }

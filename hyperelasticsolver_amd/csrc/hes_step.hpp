@@ -419,18 +419,20 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
       }
     }
     if (__any(jump)) {
-      // every lane loads (from a cell index clamped into the problem) and
-      // then selects, so the 30 loads are issued together rather than one
-      // masked load at a time; every lane is active (the branch is
-      // wave-uniform) and lane 0's update is never stored
-      const long long ic = i < a.lo ? a.lo : (i >= a.hi ? a.hi - 1 : i);
+      // only the lanes of jump faces load, all 30 rows in one masked region
+      // (so the loads issue together), and each touches only the cache lines
+      // of its own face; the shifts then run with every lane active (the
+      // branch is wave-uniform) and lane 0's update is never stored
       double d[NC];
 #pragma unroll
-      for (int r = 0; r < NC; ++r) d[r] = a.dpath[r * a.pitch + ic];
+      for (int r = 0; r < NC; ++r) d[r] = 0.0;
+      if (jump) {
+#pragma unroll
+        for (int r = 0; r < NC; ++r) d[r] = a.dpath[r * a.pitch + i];
+      }
 #pragma unroll
       for (int r = 0; r < NC; ++r) {
-        const double dmp = jump ? d[r] : 0.0;
-        upd[r] -= dtdx * (dmp + lane_up_z(dmp));  // D- of face i+1/2 + D+ of face i-1/2
+        upd[r] -= dtdx * (d[r] + lane_up_z(d[r]));  // D- of face i+1/2 + D+ of face i-1/2
         pin(upd[r]);
       }
     }
@@ -479,21 +481,25 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
       for (int r = 0; r < NC; ++r) stream(r, f[r]);
     }
     if (NPH == 2 && __any(jump)) {  // loads and shifts as in the Rusanov branch, a phase at a time
-      const long long ic = i < a.lo ? a.lo : (i >= a.hi ? a.hi - 1 : i);
 #pragma unroll
       for (int p = 0; p < NPH; ++p) {
         double dm[15], dp[15];
 #pragma unroll
         for (int k = 0; k < 15; ++k) {
-          dm[k] = a.dpath[(15 * p + k) * a.pitch + ic];
-          dp[k] = a.dpath[(NC + 15 * p + k) * a.pitch + ic];
+          dm[k] = 0.0;
+          dp[k] = 0.0;
+        }
+        if (jump) {
+#pragma unroll
+          for (int k = 0; k < 15; ++k) {
+            dm[k] = a.dpath[(15 * p + k) * a.pitch + i];
+            dp[k] = a.dpath[(NC + 15 * p + k) * a.pitch + i];
+          }
         }
 #pragma unroll
         for (int k = 0; k < 15; ++k) {
           const int r = 15 * p + k;
-          const double ddm = jump ? dm[k] : 0.0;
-          const double ddp = jump ? dp[k] : 0.0;
-          upd[r] -= dtdx * (ddm + lane_up_z(ddp));
+          upd[r] -= dtdx * (dm[k] + lane_up_z(dp[k]));
           pin(upd[r]);
         }
       }
