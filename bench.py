@@ -214,6 +214,7 @@ def time_batched_two_phase(nprob=512, n=4096, steps=20, loop_steps=3, eager_only
                 b.run(1, mode="eager")
                 out[flux] = {"mode": "eager", "ms_per_step": b.run(steps, mode="eager") / steps}
                 continue
+            tuned = b.autotune(steps=3)
             b.run(2, mode="graph")
             ms = b.run(steps, mode="graph") / steps
             loop_ms = b.run(loop_steps, mode="loop") / loop_steps
@@ -221,7 +222,8 @@ def time_batched_two_phase(nprob=512, n=4096, steps=20, loop_steps=3, eager_only
                 raise RuntimeError("non-finite batched two-phase state")
         out[flux] = {"ms_per_step": ms, "value": nprob * n / (ms / 1e3),
                      "hbm_frac": BYTES_PER_CELL_2PH * nprob * n / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
-                     "loop_ms_per_step": loop_ms, "speedup_vs_one_problem_per_launch": loop_ms / ms}
+                     "loop_ms_per_step": loop_ms, "speedup_vs_one_problem_per_launch": loop_ms / ms,
+                     "autotune": tuned}
     return out
 
 
