@@ -194,7 +194,8 @@ __global__ __launch_bounds__(MARK_BLK) void k_bmark2(BatchArgs a) {
 }
 
 // The path terms of the listed jump faces with each face's own materials, 16
-// lanes per face (as the one-problem k_path2 / k_path2_hll).
+// lanes per face for Rusanov and 8 for HLL (as the one-problem k_path2 /
+// k_path2_hll).
 template <int FAST>
 __global__ __launch_bounds__(64) void k_bpath2(BatchArgs a) {
   const unsigned long long cnt = min(a.jcnt[a.jslot], (unsigned long long)a.jcap);
@@ -208,11 +209,11 @@ __global__ __launch_bounds__(64) void k_bpath2(BatchArgs a) {
 template <int FAST>
 __global__ __launch_bounds__(64) void k_bpath2_hll(BatchArgs a) {
   const unsigned long long cnt = min(a.jcnt[a.jslot], (unsigned long long)a.jcap);
-  const unsigned long long stride = (unsigned long long)gridDim.x * 4;
-  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 4; j < cnt;
+  const unsigned long long stride = (unsigned long long)gridDim.x * 8;  // 8 faces per wave
+  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 3; j < cnt;
        j += stride)
     face_path2_hll<FAST>(a.mats + 2 * a.jprob[j], a.Qin, a.Cin, (double*)a.dpath, a.jlist[j],
-                         a.pitch, threadIdx.x & 15);
+                         a.pitch, threadIdx.x & 7);
 }
 
 // Sound speeds and per-problem CFL maximum of the current state.

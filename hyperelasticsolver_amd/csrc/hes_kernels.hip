@@ -176,15 +176,16 @@ __global__ __launch_bounds__(64) void k_path2(StepArgs a) {
     face_path2<FAST>(a.mat, a.Qin, (double*)a.dpath, a.jlist[j], a.pitch, threadIdx.x & 15);
 }
 
+// (the HLL form: a group of 8 lanes per face, eight faces per wave)
 template <int FAST>
 __global__ __launch_bounds__(64) void k_path2_hll(StepArgs a) {
   if (!(a.tring[a.step & 1] < a.t_end)) return;
   const unsigned long long cnt = min(a.jcnt[a.step & 1], (unsigned long long)a.n - 1ull);
-  const unsigned long long stride = (unsigned long long)gridDim.x * 4;
-  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 4; j < cnt;
+  const unsigned long long stride = (unsigned long long)gridDim.x * 8;  // 8 faces per wave
+  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 3; j < cnt;
        j += stride)
     face_path2_hll<FAST>(a.mat, a.Qin, a.Cin, (double*)a.dpath, a.jlist[j], a.pitch,
-                         threadIdx.x & 15);
+                         threadIdx.x & 7);
 }
 
 // Sound speeds of every cell of a state and the max CFL speed (main.jl:204-212).

@@ -684,13 +684,25 @@ __device__ __forceinline__ void face_path2(const MatDev* mat, const double* __re
   }
 }
 
+// Sum of v[30] over the 8 lanes of a lane group (every lane of the group active).
+__device__ __forceinline__ void group8_sum(double* v) {
+#pragma unroll
+  for (int o = 4; o > 0; o >>= 1)
+#pragma unroll
+    for (int r = 0; r < 30; ++r) v[r] += __shfl_xor(v[r], o, 8);
+}
+
 // HLL: for the jump face i+1/2, D-, D+ (NumFluxes.jl:70-132) minus their
 // path-free part cl J + clr dQ, cr J - clr dQ, i.e. cl and cr times the two
 // half-path integrals through the HLL state (rows 0..29: D-, rows 30..59: D+
 // of face i+1/2).  C holds the stored sound speeds (2 rows), as the step uses
-// them.  Lanes 0..5: Gauss nodes of Q_l -> Q_r; lanes 0 and 1: the two cells'
-// fluxes, then one phase each of the mid-state eigenvalue range; lanes 0..5
-// and 6..11: Gauss nodes of Q_l -> Q_hll and Q_hll -> Q_r.
+// them.  A group of 8 lanes per face (lane g), eight faces per wave: lanes
+// 0..5 evaluate the Gauss nodes of Q_l -> Q_r; lanes 0 and 1 the two cells'
+// fluxes, then one phase each of the mid-state eigenvalue range; then the 12
+// nodes of Q_l -> Q_hll and Q_hll -> Q_r in two rounds (lanes 0..7, then
+// 0..3).  A wave runs each stage for its eight faces at once, so the stages
+// that occupy only a few lanes of a group cost half of what they would with
+// 16 lanes per face.
 template <int FAST>
 __device__ __forceinline__ void face_path2_hll(const MatDev* mat, const double* __restrict__ Q,
                                                const double* __restrict__ C,
@@ -736,17 +748,17 @@ __device__ __forceinline__ void face_path2_hll(const MatDev* mat, const double* 
     mlo = t.u[0] - c;
     mhi = t.u[0] + c;
   }
-  group16_sum(bs);
-  const double smin_l = __shfl(lo, 0, 16), smax_r = __shfl(hi, 1, 16);
-  const double mn_m = fmin(__shfl(mlo, 0, 16), __shfl(mlo, 1, 16));
-  const double mx_m = fmax(__shfl(mhi, 0, 16), __shfl(mhi, 1, 16));
+  group8_sum(bs);
+  const double smin_l = __shfl(lo, 0, 8), smax_r = __shfl(hi, 1, 8);
+  const double mn_m = fmin(__shfl(mlo, 0, 8), __shfl(mlo, 1, 8));
+  const double mx_m = fmax(__shfl(mhi, 0, 8), __shfl(mhi, 1, 8));
   const HllCoef hc = hll_coef(mn_m, mx_m, smin_l, smax_r);
   // Q_hll from J = B_int(Q_l, Q_r) + F(Q_r) - F(Q_l), as face_hll forms it
   double qh[30];
   double J0 = 0.0, J15 = 0.0;
 #pragma unroll
   for (int r = 0; r < 30; ++r) {
-    const double fl = __shfl(fx[r], 0, 16), fr = __shfl(fx[r], 1, 16);
+    const double fl = __shfl(fx[r], 0, 8), fr = __shfl(fx[r], 1, 8);
     const double J = bs[r] + (fr - fl);
     qh[r] = (qn[r] * hc.s_r - q[r] * hc.s_l - J) * hc.isd;
     if (r == 0) J0 = J;
@@ -754,30 +766,43 @@ __device__ __forceinline__ void face_path2_hll(const MatDev* mat, const double* 
   }
   qh[0] = q[0] + (hc.s_r * (qn[0] - q[0]) - J0) * hc.isd;
   qh[15] = q[15] + (hc.s_r * (qn[15] - q[15]) - J15) * hc.isd;
-  // B_int(Q_l, Q_hll) + B_int(Q_hll, Q_r)
+  // B_int(Q_l, Q_hll) + B_int(Q_hll, Q_r): node m = g (lanes 0..7), then
+  // m = g + 8 (lanes 0..3); nodes 0..5 are the first segment's
   double bb[30];
 #pragma unroll
   for (int r = 0; r < 30; ++r) bb[r] = 0.0;
-  if (g < 12) {
-    const bool first = g < 6;
-    const int k = first ? g : g - 6;
-    const double s = c_gl_s[k], oms = 1.0 - s;
-    double psi[30];
+#pragma unroll 1
+  for (int round = 0; round < 2; ++round) {
+    const int m = g + 8 * round;
+    if (m < 12) {
+      const bool first = m < 6;
+      const int k = first ? m : m - 6;
+      const double s = c_gl_s[k], oms = 1.0 - s;
+      double psi[30];
 #pragma unroll
-    for (int r = 0; r < 30; ++r)
-      psi[r] = (first ? q[r] : qh[r]) * oms + (first ? qh[r] : qn[r]) * s;
-    const double da0 = first ? qh[0] - q[0] : qn[0] - qh[0];
-    const double da1 = first ? qh[15] - q[15] : qn[15] - qh[15];
-    bnode2<FAST>(mat, psi, c_gl_w[k], da0, da1, bb);
+      for (int r = 0; r < 30; ++r)
+        psi[r] = (first ? q[r] : qh[r]) * oms + (first ? qh[r] : qn[r]) * s;
+      const double da0 = first ? qh[0] - q[0] : qn[0] - qh[0];
+      const double da1 = first ? qh[15] - q[15] : qn[15] - qh[15];
+      double nb[30];
+      bnode2<FAST>(mat, psi, c_gl_w[k], da0, da1, nb);
+#pragma unroll
+      for (int r = 0; r < 30; ++r) bb[r] += nb[r];
+    }
   }
-  group16_sum(bb);
-  if (g < 15) {
-    double lo, hi;
-    rows_of_lane(bb, g, lo, hi);
-    D[g * pitch + i] = hc.cl * lo;
-    D[(30 + g) * pitch + i] = hc.cr * lo;
-    D[(g + 15) * pitch + i] = hc.cl * hi;
-    D[(45 + g) * pitch + i] = hc.cr * hi;
+  group8_sum(bb);
+  // lane g stores rows g, g + 8, g + 16 and g + 24 (< 30)
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double v = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (8 * k + j < 30) v = g == j ? bb[8 * k + j] : v;
+    const int r = 8 * k + g;
+    if (r < 30) {
+      D[r * pitch + i] = hc.cl * v;
+      D[(30 + r) * pitch + i] = hc.cr * v;
+    }
   }
 }
 
