@@ -609,7 +609,8 @@ __device__ __forceinline__ unsigned long long block_append(bool flag, unsigned l
 // w b^p(psi) da_p in the rows of phase block p, where psi is the state at the
 // node, b^p the non-zero column of B's block p and da_p the volume-fraction
 // jump of the path.  The integral is the sum over the nodes.
-template <int FAST>
+// ACC: add the share to out instead of writing it.
+template <int FAST, bool ACC = false>
 __device__ __forceinline__ void bnode2(const MatDev* mat, const double* psi, double w, double da0,
                                        double da1, double* out) {
   BCol b;
@@ -618,14 +619,15 @@ __device__ __forceinline__ void bnode2(const MatDev* mat, const double* psi, dou
   for (int p = 0; p < 2; ++p) {
     const double wd = w * (p == 0 ? da0 : da1);
     double* o = out + 15 * p;
-    o[0] = wd * b.uI0;
-    o[1] = 0.0;
-    o[2] = wd * b.sI[0];
-    o[3] = wd * b.sI[1];
-    o[4] = wd * b.sI[2];
-    o[5] = wd * b.sIu;
+    auto put = [&](int r, double v) { o[r] = ACC ? o[r] + v : v; };
+    put(0, wd * b.uI0);
+    if (!ACC) o[1] = 0.0;
+    put(2, wd * b.sI[0]);
+    put(3, wd * b.sI[1]);
+    put(4, wd * b.sI[2]);
+    put(5, wd * b.sIu);
 #pragma unroll
-    for (int r = 0; r < 9; ++r) o[6 + r] = wd * b.a[p][r];
+    for (int r = 0; r < 9; ++r) put(6 + r, wd * b.a[p][r]);
   }
 }
 
@@ -684,30 +686,37 @@ __device__ __forceinline__ void face_path2(const MatDev* mat, const double* __re
   }
 }
 
-// Sum of v[30] over the 8 lanes of a lane group (every lane of the group active).
-__device__ __forceinline__ void group8_sum(double* v) {
+// Sum of v[30] over the GL lanes of a lane group (every lane of the group active).
+template <int GL>
+__device__ __forceinline__ void group_sum(double* v) {
 #pragma unroll
-  for (int o = 4; o > 0; o >>= 1)
+  for (int o = GL / 2; o > 0; o >>= 1)
 #pragma unroll
-    for (int r = 0; r < 30; ++r) v[r] += __shfl_xor(v[r], o, 8);
+    for (int r = 0; r < 30; ++r) v[r] += __shfl_xor(v[r], o, GL);
 }
+
+// Number of lanes per jump face in the HLL path-term kernels.
+#ifndef HES_PATH_HLL_LANES
+#define HES_PATH_HLL_LANES 4
+#endif
 
 // HLL: for the jump face i+1/2, D-, D+ (NumFluxes.jl:70-132) minus their
 // path-free part cl J + clr dQ, cr J - clr dQ, i.e. cl and cr times the two
 // half-path integrals through the HLL state (rows 0..29: D-, rows 30..59: D+
 // of face i+1/2).  C holds the stored sound speeds (2 rows), as the step uses
-// them.  A group of 8 lanes per face (lane g), eight faces per wave: lanes
-// 0..5 evaluate the Gauss nodes of Q_l -> Q_r; lanes 0 and 1 the two cells'
-// fluxes, then one phase each of the mid-state eigenvalue range; then the 12
-// nodes of Q_l -> Q_hll and Q_hll -> Q_r in two rounds (lanes 0..7, then
-// 0..3).  A wave runs each stage for its eight faces at once, so the stages
-// that occupy only a few lanes of a group cost half of what they would with
-// 16 lanes per face.
-template <int FAST>
+// them.  A group of GL lanes per face (lane g), 64 / GL faces per wave: the
+// Gauss nodes of Q_l -> Q_r over the group's lanes (in rounds of GL); lanes 0
+// and 1 the two cells' fluxes, then one phase each of the mid-state
+// eigenvalue range; then the 12 nodes of Q_l -> Q_hll and Q_hll -> Q_r, again
+// in rounds.  A wave runs each stage for all its faces at once, so with few
+// lanes per face the stages that occupy only two lanes of a group cost
+// little, and the node rounds keep every lane busy.
+template <int FAST, int GL = HES_PATH_HLL_LANES>
 __device__ __forceinline__ void face_path2_hll(const MatDev* mat, const double* __restrict__ Q,
                                                const double* __restrict__ C,
                                                double* __restrict__ D, long long i,
                                                long long pitch, int g) {
+  static_assert(GL >= 2 && GL <= 16 && (GL & (GL - 1)) == 0, "lanes per face");
   double q[30], qn[30], bs[30];
 #pragma unroll
   for (int r = 0; r < 30; ++r) {
@@ -715,27 +724,19 @@ __device__ __forceinline__ void face_path2_hll(const MatDev* mat, const double* 
     qn[r] = Q[r * pitch + i + 1];
     bs[r] = 0.0;
   }
-  // B_int(Q_l, Q_r)
-  if (g < 6) {
-    const double s = c_gl_s[g], oms = 1.0 - s;
-    double psi[30];
+  // B_int(Q_l, Q_r): node m = g + GL round
+#pragma unroll 1
+  for (int round = 0; round * GL < 6; ++round) {
+    const int m = g + GL * round;
+    if (m < 6) {
+      const double s = c_gl_s[m], oms = 1.0 - s;
+      double psi[30];
 #pragma unroll
-    for (int r = 0; r < 30; ++r) psi[r] = q[r] * oms + qn[r] * s;
-    bnode2<FAST>(mat, psi, c_gl_w[g], qn[0] - q[0], qn[15] - q[15], bs);
+      for (int r = 0; r < 30; ++r) psi[r] = q[r] * oms + qn[r] * s;
+      bnode2<FAST, true>(mat, psi, c_gl_w[m], qn[0] - q[0], qn[15] - q[15], bs);
+    }
   }
-  // F(Q_l) on lane 0, F(Q_r) on lane 1, with the speed bounds cell_flux forms
-  double fx[30], lo = 0.0, hi = 0.0;
-#pragma unroll
-  for (int r = 0; r < 30; ++r) fx[r] = 0.0;
-  if (g < 2) {
-    double st[30], cs[2];
-#pragma unroll
-    for (int r = 0; r < 30; ++r) st[r] = g == 0 ? q[r] : qn[r];
-#pragma unroll
-    for (int p = 0; p < 2; ++p) cs[p] = C[p * pitch + i + g];
-    int bad;
-    cell_flux<2, FAST>(mat, st, cs, fx, lo, hi, bad);
-  }
+  group_sum<GL>(bs);
   // eigenvalue range of the mid state (Q_l + Q_r) / 2, phase g on lane g
   double mlo = 0.0, mhi = 0.0;
   if (g < 2) {
@@ -748,32 +749,52 @@ __device__ __forceinline__ void face_path2_hll(const MatDev* mat, const double* 
     mlo = t.u[0] - c;
     mhi = t.u[0] + c;
   }
-  group8_sum(bs);
-  const double smin_l = __shfl(lo, 0, 8), smax_r = __shfl(hi, 1, 8);
-  const double mn_m = fmin(__shfl(mlo, 0, 8), __shfl(mlo, 1, 8));
-  const double mx_m = fmax(__shfl(mhi, 0, 8), __shfl(mhi, 1, 8));
-  const HllCoef hc = hll_coef(mn_m, mx_m, smin_l, smax_r);
-  // Q_hll from J = B_int(Q_l, Q_r) + F(Q_r) - F(Q_l), as face_hll forms it
-  double qh[30];
-  double J0 = 0.0, J15 = 0.0;
+  // F(Q_l) on lane 0, F(Q_r) on lane 1, a phase at a time, with the speed
+  // bounds cell_flux forms; each phase's flux difference joins the path
+  // integral at once: J = B_int(Q_l, Q_r) + F(Q_r) - F(Q_l) (in bs)
+  double lo = 0.0, hi = 0.0;
 #pragma unroll
-  for (int r = 0; r < 30; ++r) {
-    const double fl = __shfl(fx[r], 0, 8), fr = __shfl(fx[r], 1, 8);
-    const double J = bs[r] + (fr - fl);
-    qh[r] = (qn[r] * hc.s_r - q[r] * hc.s_l - J) * hc.isd;
-    if (r == 0) J0 = J;
-    if (r == 15) J15 = J;
+  for (int p = 0; p < 2; ++p) {
+    double fp[15];
+#pragma unroll
+    for (int r = 0; r < 15; ++r) fp[r] = 0.0;
+    if (g < 2) {
+      double qp[15];
+#pragma unroll
+      for (int r = 0; r < 15; ++r) qp[r] = g == 0 ? q[15 * p + r] : qn[15 * p + r];
+      Thermo t;
+      thermo<false, FAST>(mat[p], qp[0], qp + 1, t);
+      phase_flux(mat[p], qp[0], t, qp + 6, fp + 1);
+      const double cs = C[p * pitch + i + g];
+      const double l = t.u[0] - cs, h = t.u[0] + cs;
+      lo = p == 0 ? l : fmin(lo, l);
+      hi = p == 0 ? h : fmax(hi, h);
+    }
+#pragma unroll
+    for (int r = 0; r < 15; ++r) {
+      const double fl = __shfl(fp[r], 0, GL), fr = __shfl(fp[r], 1, GL);
+      bs[15 * p + r] = bs[15 * p + r] + (fr - fl);
+    }
   }
+  const double smin_l = __shfl(lo, 0, GL), smax_r = __shfl(hi, 1, GL);
+  const double mn_m = fmin(__shfl(mlo, 0, GL), __shfl(mlo, 1, GL));
+  const double mx_m = fmax(__shfl(mhi, 0, GL), __shfl(mhi, 1, GL));
+  const HllCoef hc = hll_coef(mn_m, mx_m, smin_l, smax_r);
+  // Q_hll from J, as face_hll forms it
+  double qh[30];
+#pragma unroll
+  for (int r = 0; r < 30; ++r) qh[r] = (qn[r] * hc.s_r - q[r] * hc.s_l - bs[r]) * hc.isd;
+  const double J0 = bs[0], J15 = bs[15];
   qh[0] = q[0] + (hc.s_r * (qn[0] - q[0]) - J0) * hc.isd;
   qh[15] = q[15] + (hc.s_r * (qn[15] - q[15]) - J15) * hc.isd;
-  // B_int(Q_l, Q_hll) + B_int(Q_hll, Q_r): node m = g (lanes 0..7), then
-  // m = g + 8 (lanes 0..3); nodes 0..5 are the first segment's
+  // B_int(Q_l, Q_hll) + B_int(Q_hll, Q_r): node m = g + GL round; nodes 0..5
+  // are the first segment's
   double bb[30];
 #pragma unroll
   for (int r = 0; r < 30; ++r) bb[r] = 0.0;
 #pragma unroll 1
-  for (int round = 0; round < 2; ++round) {
-    const int m = g + 8 * round;
+  for (int round = 0; round * GL < 12; ++round) {
+    const int m = g + GL * round;
     if (m < 12) {
       const bool first = m < 6;
       const int k = first ? m : m - 6;
@@ -784,21 +805,18 @@ __device__ __forceinline__ void face_path2_hll(const MatDev* mat, const double* 
         psi[r] = (first ? q[r] : qh[r]) * oms + (first ? qh[r] : qn[r]) * s;
       const double da0 = first ? qh[0] - q[0] : qn[0] - qh[0];
       const double da1 = first ? qh[15] - q[15] : qn[15] - qh[15];
-      double nb[30];
-      bnode2<FAST>(mat, psi, c_gl_w[k], da0, da1, nb);
-#pragma unroll
-      for (int r = 0; r < 30; ++r) bb[r] += nb[r];
+      bnode2<FAST, true>(mat, psi, c_gl_w[k], da0, da1, bb);
     }
   }
-  group8_sum(bb);
-  // lane g stores rows g, g + 8, g + 16 and g + 24 (< 30)
+  group_sum<GL>(bb);
+  // lane g stores rows g, g + GL, g + 2 GL, ... (< 30)
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k * GL < 30; ++k) {
     double v = 0.0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (8 * k + j < 30) v = g == j ? bb[8 * k + j] : v;
-    const int r = 8 * k + g;
+    for (int j = 0; j < GL; ++j)
+      if (GL * k + j < 30) v = g == j ? bb[GL * k + j] : v;
+    const int r = GL * k + g;
     if (r < 30) {
       D[r * pitch + i] = hc.cl * v;
       D[(30 + r) * pitch + i] = hc.cr * v;
