@@ -194,26 +194,28 @@ __global__ __launch_bounds__(MARK_BLK) void k_bmark2(BatchArgs a) {
 }
 
 // The path terms of the listed jump faces with each face's own materials, 16
-// lanes per face for Rusanov and 8 for HLL (as the one-problem k_path2 /
+// lanes per face for Rusanov and HES_PATH_HLL_LANES for HLL (as the one-problem k_path2 /
 // k_path2_hll).
 template <int FAST>
 __global__ __launch_bounds__(64) void k_bpath2(BatchArgs a) {
   const unsigned long long cnt = min(a.jcnt[a.jslot], (unsigned long long)a.jcap);
-  const unsigned long long stride = (unsigned long long)gridDim.x * 4;
-  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 4; j < cnt;
+  constexpr int GL = HES_PATH_LANES;
+  const unsigned long long stride = (unsigned long long)gridDim.x * (64 / GL);
+  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) / GL; j < cnt;
        j += stride)
     face_path2<FAST>(a.mats + 2 * a.jprob[j], a.Qin, (double*)a.dpath, a.jlist[j], a.pitch,
-                     threadIdx.x & 15);
+                     threadIdx.x & (GL - 1));
 }
 
 template <int FAST>
 __global__ __launch_bounds__(64) void k_bpath2_hll(BatchArgs a) {
   const unsigned long long cnt = min(a.jcnt[a.jslot], (unsigned long long)a.jcap);
-  const unsigned long long stride = (unsigned long long)gridDim.x * 8;  // 8 faces per wave
-  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 3; j < cnt;
+  constexpr int GL = HES_PATH_HLL_LANES;
+  const unsigned long long stride = (unsigned long long)gridDim.x * (64 / GL);
+  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) / GL; j < cnt;
        j += stride)
     face_path2_hll<FAST>(a.mats + 2 * a.jprob[j], a.Qin, a.Cin, (double*)a.dpath, a.jlist[j],
-                         a.pitch, threadIdx.x & 7);
+                         a.pitch, threadIdx.x & (GL - 1));
 }
 
 // Sound speeds and per-problem CFL maximum of the current state.
@@ -451,14 +453,14 @@ void launch_batch(hes_batch* h, const BatchArgs& a) {
     const int tiles = a.tile_end - a.tile_begin;
     if (tiles <= 0) return;
     // jump faces first (a light scan), then their path terms over a fixed
-    // grid (as many waves as stay resident: three per SIMD for Rusanov, one
+    // grid (as many waves as stay resident: two per SIMD for Rusanov, one
     // for HLL), then the step
     hipStream_t st = h->stream;
     hipLaunchKernelGGL(k_bmark2, dim3((unsigned)((tiles + MARK_BLK / 64 - 1) / (MARK_BLK / 64))),
                        dim3(MARK_BLK), 0, st, a);
     const long long nbf = ((long long)tiles * TILE + 254) / 256;
     const unsigned nbl = (unsigned)std::min<long long>(
-        nbf, h->flux != HES_FLUX_HLL ? 3 * 1024 : 1024);
+        nbf, h->flux != HES_FLUX_HLL ? 2 * 1024 : 1024);
     if (h->flux != HES_FLUX_HLL) {
       if (h->fast) hipLaunchKernelGGL(k_bpath2<1>, dim3(nbl), dim3(64), 0, st, a);
       else hipLaunchKernelGGL(k_bpath2<0>, dim3(nbl), dim3(64), 0, st, a);

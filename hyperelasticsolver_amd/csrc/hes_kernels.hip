@@ -163,29 +163,31 @@ __global__ __launch_bounds__(MARK_BLK) void k_mark2(StepArgs a) {
   if (jump && pos < (unsigned long long)a.n) a.jlist[pos] = i;  // at most n - 1 faces
 }
 
-// The path terms of the listed jump faces over a fixed grid: a group of 16
-// lanes per face, four faces per wave (the faces without a jump never reach
-// this kernel).
+// The path terms of the listed jump faces over a fixed grid: a group of
+// HES_PATH_LANES lanes per face, 64 / HES_PATH_LANES faces per wave (the faces
+// without a jump never reach this kernel).
 template <int FAST>
 __global__ __launch_bounds__(64) void k_path2(StepArgs a) {
   if (!(a.tring[a.step & 1] < a.t_end)) return;
   const unsigned long long cnt = min(a.jcnt[a.step & 1], (unsigned long long)a.n - 1ull);
-  const unsigned long long stride = (unsigned long long)gridDim.x * 4;
-  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 4; j < cnt;
+  constexpr int GL = HES_PATH_LANES;
+  const unsigned long long stride = (unsigned long long)gridDim.x * (64 / GL);
+  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) / GL; j < cnt;
        j += stride)
-    face_path2<FAST>(a.mat, a.Qin, (double*)a.dpath, a.jlist[j], a.pitch, threadIdx.x & 15);
+    face_path2<FAST>(a.mat, a.Qin, (double*)a.dpath, a.jlist[j], a.pitch, threadIdx.x & (GL - 1));
 }
 
-// (the HLL form: a group of 8 lanes per face, eight faces per wave)
+// (the HLL form: HES_PATH_HLL_LANES lanes per face)
 template <int FAST>
 __global__ __launch_bounds__(64) void k_path2_hll(StepArgs a) {
   if (!(a.tring[a.step & 1] < a.t_end)) return;
   const unsigned long long cnt = min(a.jcnt[a.step & 1], (unsigned long long)a.n - 1ull);
-  const unsigned long long stride = (unsigned long long)gridDim.x * 8;  // 8 faces per wave
-  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) >> 3; j < cnt;
+  constexpr int GL = HES_PATH_HLL_LANES;
+  const unsigned long long stride = (unsigned long long)gridDim.x * (64 / GL);
+  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) / GL; j < cnt;
        j += stride)
     face_path2_hll<FAST>(a.mat, a.Qin, a.Cin, (double*)a.dpath, a.jlist[j], a.pitch,
-                         threadIdx.x & 7);
+                         threadIdx.x & (GL - 1));
 }
 
 // Sound speeds of every cell of a state and the max CFL speed (main.jl:204-212).
@@ -506,10 +508,10 @@ void launch_step_t(const StepArgs& a, long long nchunks, bool fast, hipStream_t 
     const unsigned nbp = (unsigned)((a.n + MARK_BLK - 1) / MARK_BLK);
     hipLaunchKernelGGL(k_mark2, dim3(nbp), dim3(MARK_BLK), 0, s, a);
     // 4 faces per block; as many blocks as the kernels' occupancy keeps
-    // resident (1024 SIMDs): three waves per SIMD for Rusanov, one for HLL
+    // resident (1024 SIMDs): two waves per SIMD for Rusanov, one for HLL
     const long long nbf = (a.n + 254) / 256;
     if (FLUX == FLUX_RUSANOV) {
-      const unsigned nbl = (unsigned)std::min<long long>(nbf, 3 * 1024);
+      const unsigned nbl = (unsigned)std::min<long long>(nbf, 2 * 1024);
       if (fast) hipLaunchKernelGGL(k_path2<1>, dim3(nbl), dim3(64), 0, s, a);
       else hipLaunchKernelGGL(k_path2<0>, dim3(nbl), dim3(64), 0, s, a);
     } else {

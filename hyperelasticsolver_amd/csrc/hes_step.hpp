@@ -651,38 +651,58 @@ __device__ __forceinline__ void rows_of_lane(const double* v, int g, double& lo,
   }
 }
 
-// The jump faces' path terms are computed by a group of 16 lanes per face
-// (lane g of the group), one quadrature node per lane: the nodes evaluate the
-// same code on different data, which a wave runs in parallel.
+// The jump faces' path terms are computed by a group of lanes per face (lane
+// g of the group), the quadrature nodes spread over the group's lanes: the
+// nodes evaluate the same code on different data, which a wave runs in
+// parallel for all its faces.
+
+// Number of lanes per jump face in the Rusanov path-term kernels.
+#ifndef HES_PATH_LANES
+#define HES_PATH_LANES 2
+#endif
 
 // Rusanov: half path integral 1/2 int B dpsi of the jump face i+1/2
-// (0 <= i < n - 1) of a two-phase state (D- = D+ of NumFluxes.jl:35-59),
-// Gauss-Lobatto nodes on lanes 0..5; only jump faces are written (the step
-// reads only those).
-template <int FAST>
+// (0 <= i < n - 1) of a two-phase state (D- = D+ of NumFluxes.jl:35-59): the
+// 6 Gauss-Lobatto nodes over the group's GL lanes (node m = g + GL round), a
+// group sum, and lane g stores rows g, g + GL, ...; only jump faces are
+// written (the step reads only those).
+template <int FAST, int GL = HES_PATH_LANES>
 __device__ __forceinline__ void face_path2(const MatDev* mat, const double* __restrict__ Q,
                                            double* __restrict__ D, long long i, long long pitch,
                                            int g) {
-  double q[30], qn[30], d[30];
+  double d[30];
 #pragma unroll
-  for (int r = 0; r < 30; ++r) {
-    q[r] = Q[r * pitch + i];
-    qn[r] = Q[r * pitch + i + 1];
-    d[r] = 0.0;
-  }
-  if (g < 6) {
-    const double s = c_lo_s[g], oms = 1.0 - s;
-    double psi[30];
+  for (int r = 0; r < 30; ++r) d[r] = 0.0;
+#pragma unroll 1
+  for (int round = 0; round * GL < 6; ++round) {
+    const int m = g + GL * round;
+    if (m < 6) {
+      // the two cell states are read in every round (from L2) rather than
+      // held across the rounds: an offset the compiler cannot see through
+      // keeps it from hoisting the loads out of the loop
+      int ofs = 0;
+      asm volatile("" : "+v"(ofs));
+      const double* __restrict__ ql = Q + i + ofs;
+      const double s = c_lo_s[m], oms = 1.0 - s;
+      double psi[30];
 #pragma unroll
-    for (int r = 0; r < 30; ++r) psi[r] = q[r] * oms + qn[r] * s;
-    bnode2<FAST>(mat, psi, c_lo_w[g], qn[0] - q[0], qn[15] - q[15], d);
+      for (int r = 0; r < 30; ++r) psi[r] = ql[r * pitch] * oms + ql[r * pitch + 1] * s;
+      bnode2<FAST, true>(mat, psi, c_lo_w[m], ql[1] - ql[0], ql[15 * pitch + 1] - ql[15 * pitch],
+                         d);
+    }
   }
-  group16_sum(d);
-  if (g < 15) {
-    double lo, hi;
-    rows_of_lane(d, g, lo, hi);
-    D[g * pitch + i] = 0.5 * lo;
-    D[(g + 15) * pitch + i] = 0.5 * hi;
+#pragma unroll
+  for (int o = GL / 2; o > 0; o >>= 1)
+#pragma unroll
+    for (int r = 0; r < 30; ++r) d[r] += __shfl_xor(d[r], o, GL);
+#pragma unroll
+  for (int k = 0; k * GL < 30; ++k) {
+    double v = 0.0;
+#pragma unroll
+    for (int j = 0; j < GL; ++j)
+      if (GL * k + j < 30) v = g == j ? d[GL * k + j] : v;
+    const int r = GL * k + g;
+    if (r < 30) D[r * pitch + i] = 0.5 * v;
   }
 }
 
