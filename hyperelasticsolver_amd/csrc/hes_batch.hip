@@ -193,9 +193,9 @@ __global__ __launch_bounds__(MARK_BLK) void k_bmark2(BatchArgs a) {
   }
 }
 
-// The path terms of the listed jump faces with each face's own materials, 16
-// lanes per face for Rusanov and HES_PATH_HLL_LANES for HLL (as the one-problem k_path2 /
-// k_path2_hll).
+// The path terms of the listed jump faces with each face's own materials,
+// HES_PATH_LANES lanes per face for Rusanov and HES_PATH_HLL_LANES for HLL (as
+// the one-problem k_path2 / k_path2_hll).
 template <int FAST>
 __global__ __launch_bounds__(64) void k_bpath2(BatchArgs a) {
   const unsigned long long cnt = min(a.jcnt[a.jslot], (unsigned long long)a.jcap);
