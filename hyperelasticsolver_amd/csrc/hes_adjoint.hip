@@ -316,38 +316,100 @@ struct AdjStepArgs {
 #define HES_ADJ_WAVES_PER_EU 1
 #endif
 
+constexpr int PROWS = 2 * NQ + 1;  // rows of a tile in LDS: Q (14), its sound speed, Qbar_{s+1} (14)
+typedef __attribute__((address_space(3))) void lds_t;
+
+// Copies the PROWS rows of tile w (cells w*TILE-1 .. w*TILE+62, clamped to
+// [0, n)) from HBM into a wave's LDS buffer at byte address `lds`, without
+// passing through registers: each instruction moves 4 bytes per lane, lane l
+// taking half l&1 of cell 32h + l/2 of the tile, so a row takes two
+// instructions and lands in doubles 64 r .. 64 r + 63 of the buffer in cell
+// order.  Byte offsets within a row are 32-bit (n < 2^28 is checked at
+// creation).
+__device__ __forceinline__ void tile_prefetch(const AdjStepArgs& a, long long w, int lane,
+                                              unsigned lds) {
+  const long long i0 = w * TILE - 1;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    long long c = i0 + 32 * h + (lane >> 1);
+    c = c < 0 ? 0 : (c >= a.n ? a.n - 1 : c);
+    unsigned off = (unsigned)c * 8u + (unsigned)(lane & 1) * 4u;
+    // fresh values per call: the row addresses are formed here, one at a
+    // time, rather than hoisted out of the tile loop into 58 registers
+    const char* q = (const char*)a.Q;
+    const char* qb = (const char*)a.Qb_in;
+    unsigned d = lds + 256u * h;
+    asm volatile("" : "+v"(off), "+s"(q), "+s"(qb), "+s"(d));
+    const unsigned long long rowb = (unsigned long long)a.pitch * 8ull;
+#pragma unroll
+    for (int r = 0; r < PROWS; ++r) {
+      const char* src = r <= NQ ? q + r * rowb : qb + (r - NQ - 1) * rowb;
+      __builtin_amdgcn_global_load_lds((const void*)(src + off), (lds_t*)(size_t)(d + 512u * r),
+                                       4, 0, 0);
+    }
+  }
+}
+
+// Persistent waves, one per SIMD (a wave holds 256 VGPRs + 78 AGPRs, so a
+// forward wave of the recompute fits beside it): wave k of the grid takes
+// tiles k, k + W, k + 2W, ...  While it evaluates one tile, the next one it
+// will evaluate streams into its LDS buffer, so only the first tile's loads
+// are waited for.  The block partials of dtdxbar and (b0bar, c0bar) sum the
+// block's tiles in a fixed order.
 __global__ __launch_bounds__(ABLK) __attribute__((amdgpu_waves_per_eu(HES_ADJ_WAVES_PER_EU)))
 void k_adj_step(AdjStepArgs a) {
+  __shared__ double sbuf[ABLK / 64][PROWS * 64];
   const int lane = threadIdx.x & 63;
-  const long long wave = (xcd_block() * ABLK + threadIdx.x) >> 6;
-  const long long i = wave * TILE + lane - 1;
-  const bool inr = i >= 0 && i < a.n;
-  const bool owned = inr && lane >= 1 && lane <= TILE;
+  const int wl = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double* buf = sbuf[wl];
+  const unsigned lds = (unsigned)(size_t)(lds_t*)buf;
+  const long long W = (long long)gridDim.x * (ABLK / 64);
   const long long pitch = a.pitch;
   const long long is1 = a.has_next ? a.istar[a.s + 1] : -2;
-  const bool hit = is1 >= wave * TILE - 1 && is1 <= wave * TILE + TILE;  // wave-uniform
-  bool skip = wave >= a.nw;  // tiles past the last cell own nothing
-  if (!skip && a.sparse && !hit) {
-    skip = a.zin[wave] && (wave == 0 || a.zin[wave - 1]) && (wave + 1 >= a.nw || a.zin[wave + 1]);
-    if (skip) {
-      if (!a.zout[wave] && owned) {
+  // The first tile from w on (in steps of W) that must be evaluated; the
+  // skipped ones on the way get their zero cotangent.  The zero flags of
+  // Qbar_{s+1} are read-only in this kernel: read through the constant
+  // address space, they load into scalar registers, and no wait for them
+  // waits for vector memory.
+  typedef const __attribute__((address_space(4))) int ci32_t;
+  ci32_t* zin = (ci32_t*)a.zin;
+  auto next_tile = [&](long long w) -> long long {
+    for (; w < a.nw; w += W) {
+      const bool hit = is1 >= w * TILE - 1 && is1 <= w * TILE + TILE;
+      if (!a.sparse || hit) return w;
+      const bool skip = zin[w] && (w == 0 || zin[w - 1]) && (w + 1 >= a.nw || zin[w + 1]);
+      if (!skip) return w;
+      const long long i = w * TILE + lane - 1;
+      if (!a.zout[w] && i >= 0 && i < a.n && lane >= 1 && lane <= TILE) {
 #pragma unroll
         for (int r = 0; r < NQ; ++r) a.Qb_out[r * pitch + i] = 0.0;
       }
-      if (lane == 0) a.zout[wave] = 1;
+      if (lane == 0) a.zout[w] = 1;
     }
-  }
+    return w;
+  };
   double dts = 0.0, th0 = 0.0, th1 = 0.0;
-  if (!skip) {
-    const long long ic = i < 0 ? 0 : (i >= a.n ? a.n - 1 : i);
+  const double dtdx = a.cfl * rcp(lmax_of(a.lmax, a.s));  // as chunk_step forms it
+  long long w = next_tile(xcd_block() * (ABLK / 64) + wl);
+  if (w < a.nw) tile_prefetch(a, w, lane, lds);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the first tile is in LDS
+  while (w < a.nw) {
+    const long long i = w * TILE + lane - 1;
+    const bool inr = i >= 0 && i < a.n;
+    const bool owned = inr && lane >= 1 && lane <= TILE;
+    const bool hit = is1 >= w * TILE - 1 && is1 <= w * TILE + TILE;  // wave-uniform
     double q[NQ], qb[NQ];
 #pragma unroll
     for (int r = 0; r < NQ; ++r) {
-      q[r] = ld_stream(a.Q + r * pitch + ic);
-      qb[r] = inr ? ld_stream(a.Qb_in + r * pitch + ic) : 0.0;
+      q[r] = buf[64 * r + lane];
+      qb[r] = inr ? buf[64 * (NQ + 1 + r) + lane] : 0.0;
     }
     // the state's sound speeds, stored in row 14 of its slot by the forward step
-    const double cs = ld_stream(a.Q + NQ * pitch + ic);
+    const double cs = buf[64 * NQ + lane];
+    // every lane's reads are in registers before the next tile overwrites the buffer
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    const long long wn = next_tile(w + W);
+    if (wn < a.nw) tile_prefetch(a, wn, lane, lds);
     // lmax term of step s+1 at its argmax cell
     if (hit) {
       double acc = 0.0;
@@ -367,9 +429,10 @@ void k_adj_step(AdjStepArgs a) {
       }
     }
     const bool interior = inr && i > 0 && i < a.n - 1;
-    const double dtdx = a.cfl * rcp(lmax_of(a.lmax, a.s));  // as chunk_step forms it
     double ib[hes_gen::CELL_IN];
-    reverse_chunk(q, qb, interior, owned, cs, 0.5 * dtdx, a.gin, ib, dts);
+    double dtt = 0.0;
+    reverse_chunk(q, qb, interior, owned, cs, 0.5 * dtdx, a.gin, ib, dtt);
+    dts += dtt;
     bool zero = true;
     if (owned) {
 #pragma unroll
@@ -382,7 +445,12 @@ void k_adj_step(AdjStepArgs a) {
       th1 += ib[NQ + 1];
     }
     const int z = __all(zero);
-    if (lane == 0) a.zout[wave] = z;
+    if (lane == 0) a.zout[w] = z;
+    // The next tile is in LDS once at most the 14 stores above are in
+    // flight: vector memory operations retire in issue order, and a tile
+    // below nw owns at least one cell, so all 14 are issued.
+    __builtin_amdgcn_s_waitcnt(0x0F7E);  // vmcnt(14)
+    w = wn;
   }
   const double bd = block_sum<ABLK>(dts);
   const double b0s = block_sum<ABLK>(th0);
@@ -510,6 +578,7 @@ __global__ __launch_bounds__(64) void k_adj_final(
 struct hes_adjoint {
   long long n, pitch, nsteps, every, nck;
   int nb;    // blocks of the tiled kernels
+  int nbr;   // blocks of the reverse step: one per CU (persistent waves)
   int nbl;   // blocks of the one-cell-per-thread kernels
   int nw;    // tiles of TILE cells
   int sparse = 1;
@@ -710,7 +779,7 @@ hipError_t record_backward(hes_adjoint* h) {
       a.s = s;
       a.cfl = h->cfl;
       a.dx = h->dx;
-      a.nb = h->nb;
+      a.nb = h->nbr;
       a.nw = h->nw;
       a.sparse = h->sparse;
       a.has_next = s + 1 < h->nsteps ? 1 : 0;
@@ -719,7 +788,7 @@ hipError_t record_backward(hes_adjoint* h) {
       a.gin[2] = h->mat8[0];
       a.gin[3] = h->mat8[2];
       a.gin[4] = h->mat8[3];
-      hipLaunchKernelGGL(k_adj_step, dim3((unsigned)h->nb), dim3(ABLK), 0, st, a);
+      hipLaunchKernelGGL(k_adj_step, dim3((unsigned)h->nbr), dim3(ABLK), 0, st, a);
     }
     HIP_TRY_E(hipEventRecord(h->ev_done[buf], st));
   }
@@ -727,7 +796,7 @@ hipError_t record_backward(hes_adjoint* h) {
   HIP_TRY_E(hipStreamWaitEvent(st, h->ev_join, 0));
   hipLaunchKernelGGL(k_adj_q0sum, dim3((unsigned)h->nbl), dim3(LBLK), 0, st, h->qb[0], h->bounds,
                      h->nst, h->part, h->n, h->pitch);
-  hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(64), 0, st, h->part, h->nbl, h->thb, h->nb,
+  hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(64), 0, st, h->part, h->nbl, h->thb, h->nbr,
                      h->dtp, h->lmax, h->istar, h->Jlam, h->bounds, h->nst, h->prims, h->gin,
                      h->cfl, h->dx, h->sums, h->gout);
   return hipGetLastError();
@@ -766,6 +835,12 @@ int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* materi
     return fail(HES_ERR_ARG, "the adjoint supports the exponents alpha=1, beta=3, gamma=2");
   if (!(cfl > 0) || !(dx > 0)) return fail(HES_ERR_ARG, "cfl and dx must be positive");
   if (nsteps > 0x7fffffffLL) return fail(HES_ERR_ARG, "too many steps");
+  if (ncells >= (1LL << 28)) return fail(HES_ERR_ARG, "at most 2^28 - 1 cells");
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    return fail(HES_ERR_HIP, "cannot read the device's compute-unit count");
   hes_adjoint* h = new hes_adjoint();
   h->n = ncells;
   h->pitch = (ncells + 63) / 64 * 64 + 256;  // rows padded off a power-of-two stride, as the solver's
@@ -775,6 +850,7 @@ int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* materi
   const long long nwaves = (ncells + TILE - 1) / TILE;
   h->nw = (int)nwaves;
   h->nb = (int)((nwaves + ABLK / 64 - 1) / (ABLK / 64));
+  h->nbr = std::min(h->nb, cus * (256 / ABLK));
   h->nbl = (int)((ncells + LBLK - 1) / LBLK);
   h->cfl = cfl;
   h->dx = dx;

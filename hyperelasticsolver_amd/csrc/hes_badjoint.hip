@@ -339,93 +339,206 @@ struct BAdjStepArgs {
   double cfl;
   int P, has_next;
   int sparse;                        // skip tiles whose cotangent is zero on them and their halo
+  int ntiles;
 };
+
+constexpr int PROWS = 2 * NQ + 1;  // rows of a tile in LDS: Q (14), its sound speed, Qbar_{s+1} (14)
+typedef __attribute__((address_space(3))) void lds_t;
+
+// Copies the PROWS rows of tile td (cells first-1 .. first+62, clamped to its
+// problem's [lo, hi)) from HBM into the wave's LDS buffer at byte address
+// `lds`, without passing through registers: each instruction moves 4 bytes
+// per lane, lane l taking half l&1 of cell 32h + l/2 of the tile, so a row
+// takes two instructions and lands in doubles 64 r .. 64 r + 63 of the buffer
+// in cell order.  Byte offsets within a row are 32-bit (fewer than 2^28
+// cells in all is checked at creation).
+__device__ __forceinline__ void btile_prefetch(const BAdjStepArgs& a, const BTile& td, int lane,
+                                               unsigned lds) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    long long c = td.first - 1 + 32 * h + (lane >> 1);
+    c = c < td.lo ? td.lo : (c >= td.hi ? td.hi - 1 : c);
+    unsigned off = (unsigned)c * 8u + (unsigned)(lane & 1) * 4u;
+    // fresh values per call: the row addresses are formed here, one at a
+    // time, rather than hoisted out of the tile loop into 58 registers
+    const char* q = (const char*)a.Q;
+    const char* qb = (const char*)a.Qb_in;
+    unsigned d = lds + 256u * h;
+    asm volatile("" : "+v"(off), "+s"(q), "+s"(qb), "+s"(d));
+    const unsigned long long rowb = (unsigned long long)a.pitch * 8ull;
+#pragma unroll
+    for (int r = 0; r < PROWS; ++r) {
+      const char* src = r <= NQ ? q + r * rowb : qb + (r - NQ - 1) * rowb;
+      __builtin_amdgcn_global_load_lds((const void*)(src + off), (lds_t*)(size_t)(d + 512u * r),
+                                       4, 0, 0);
+    }
+  }
+}
 
 // One reverse step of every problem.  A tile whose Qbar of state s+1 is zero
 // on its owned cells and on the neighbouring tiles of its problem (and which
 // holds no argmax cell of step s+1) has Qbar of state s = 0 and contributes
 // nothing to dtdxbar or (b0bar, c0bar).
+//
+// Persistent waves, one per SIMD: wave k of the grid takes tiles k, k + W,
+// k + 2W, ...  While it evaluates one tile, the next one it will evaluate
+// streams into its LDS buffer with that tile's CFL maximum, argmax cell,
+// running (b0bar, c0bar) and material inputs, so only the first tile's loads
+// are waited for.  Every tile's arithmetic and partials are those of a wave
+// that evaluates it alone.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) void k_badj_step(
     BAdjStepArgs a) {
-  const int tile = btile();
-  const BTile td = a.tiles[tile];
-  const int p = td.p;
+  // the tile's rows, then its context: CFL maximum (dwords 0-1), argmax
+  // cell of step s+1 (2-3), running (b0bar, c0bar) (4-7), material inputs
+  // (8-17)
+  __shared__ double buf[PROWS * 64 + 32];
+  const unsigned lds = (unsigned)(size_t)(lds_t*)buf;
+  const double* ctx = buf + PROWS * 64;
   const int lane = threadIdx.x & 63;
-  const long long lo = td.lo, hi = td.hi, first = td.first;
-  const long long i = first + lane - 1;
-  const bool inr = i >= lo && i < hi;
-  const bool owned = inr && lane >= 1 && lane <= TILE;
+  const int W = (int)gridDim.x;
   const long long pitch = a.pitch;
-  const long long is1 = a.has_next ? a.istar[(a.s + 1) * a.P + p] : -2;
-  const bool hit = is1 >= first - 1 && is1 <= first + TILE;  // wave-uniform
-  bool skip = false;
-  if (a.sparse && !hit) {
-    skip = a.zin[tile] && (first == lo || a.zin[tile - 1]) && (first + TILE >= hi || a.zin[tile + 1]);
-    if (skip) {
-      if (!a.zout[tile] && owned) {
+  // The tile records, the CFL records, the argmax cells and the zero flags
+  // of Qbar_{s+1} are read-only in this kernel: read through the constant
+  // address space, they load into scalar registers, and no wait for them
+  // waits for vector memory.
+  typedef const __attribute__((address_space(4))) BTile ctile_t;
+  typedef const __attribute__((address_space(4))) long long ci64_t;
+  typedef const __attribute__((address_space(4))) int ci32_t;
+  auto record = [&](int t) -> BTile {
+    ctile_t* tr = (ctile_t*)a.tiles + t;
+    BTile r;
+    r.first = tr->first;
+    r.lo = tr->lo;
+    r.hi = tr->hi;
+    r.p = tr->p;
+    return r;
+  };
+  auto is1_of = [&](int p) -> long long {
+    return a.has_next ? ((ci64_t*)a.istar)[(a.s + 1) * a.P + p] : -2;
+  };
+  // The first tile from t on (in steps of W) that must be evaluated; the
+  // skipped ones on the way get their zero cotangent and partial.
+  auto next_tile = [&](int t) -> int {
+    for (; t < a.ntiles; t += W) {
+      if (!a.sparse) return t;
+      const BTile td = record(t);
+      const long long is1 = is1_of(td.p);
+      if (is1 >= td.first - 1 && is1 <= td.first + TILE) return t;
+      ci32_t* zin = (ci32_t*)a.zin;
+      const bool skip = zin[t] && (td.first == td.lo || zin[t - 1]) &&
+                        (td.first + TILE >= td.hi || zin[t + 1]);
+      if (!skip) return t;
+      const long long i = td.first + lane - 1;
+      if (!a.zout[t] && i >= td.lo && i < td.hi && lane >= 1 && lane <= TILE) {
 #pragma unroll
         for (int r = 0; r < NQ; ++r) a.Qb_out[r * pitch + i] = 0.0;
       }
       if (lane == 0) {
-        a.zout[tile] = 1;
-        a.dtp[tile] = 0.0;
-      }
-      return;  // the whole wave
-    }
-  }
-  double dts = 0.0, th0 = 0.0, th1 = 0.0;
-  const double* gin = a.gin + 5 * p;
-  const long long ic = i < lo ? lo : (i >= hi ? hi - 1 : i);
-  double q[NQ], qb[NQ];
-#pragma unroll
-  for (int r = 0; r < NQ; ++r) {
-    q[r] = ld_stream(a.Q + r * pitch + ic);
-    qb[r] = inr ? ld_stream(a.Qb_in + r * pitch + ic) : 0.0;
-  }
-  // the state's sound speeds, stored in row 14 of its slot by the forward step
-  const double cs = ld_stream(a.Q + NQ * pitch + ic);
-  // lmax term of step s+1 at its argmax cell
-  if (hit) {
-    double acc = 0.0;
-    for (int b = a.tfirst[p] + lane; b < a.tfirst[p + 1]; b += 64) acc += a.dtp_next[b];
-    acc = wave_sum(acc);
-    const double lm1 = lmax_of(a.lmax, (a.s + 1) * a.P + p);
-    const double dtdx1 = a.cfl * rcp(lm1);  // as chunk_step forms dt / dx
-    const double lbar = -acc * dtdx1 * rcp(lm1);
-    if (inr && i == is1) {
-      const double* J = a.Jlam + ((a.s + 1) * a.P + p) * JL;
-#pragma unroll
-      for (int r = 0; r < NQ; ++r) qb[r] += lbar * J[r];
-      if (owned) {
-        th0 += lbar * J[NQ];
-        th1 += lbar * J[NQ + 1];
+        a.zout[t] = 1;
+        a.dtp[t] = 0.0;
       }
     }
-  }
-  const bool interior = inr && i > lo && i < hi - 1;
-  const double dtdx = a.cfl * rcp(lmax_of(a.lmax, a.s * a.P + p));  // as chunk_step forms it
-  double ib[hes_gen::CELL_IN];
-  reverse_chunk(q, qb, interior, owned, cs, 0.5 * dtdx, gin, ib, dts);
-  bool zero = true;
-  if (owned) {
+    return t;
+  };
+  // The next tile's record (one scalar round trip: its rows' addresses
+  // depend on it), then its context and its rows into LDS.
+  BTile td;
+  auto fetch = [&](int t) {
+    td = record(t);
+    const char* src = (const char*)(a.lmax + a.s * a.P + td.p);
+    if (lane >= 2 && lane < 4 && a.has_next)
+      src = (const char*)(a.istar + (a.s + 1) * a.P + td.p) + 4 * (lane - 2);
+    else if (lane >= 4 && lane < 8)
+      src = (const char*)(a.thb + 2 * (long long)t) + 4 * (lane - 4);
+    else if (lane >= 8 && lane < 18)
+      src = (const char*)(a.gin + 5 * (long long)td.p) + 4 * (lane - 8);
+    else if (lane < 2)
+      src += 4 * lane;
+    unsigned dc = lds + PROWS * 512u;
+    asm volatile("" : "+s"(dc));  // the explicit waits below order the context's reads
+    __builtin_amdgcn_global_load_lds((const void*)src, (lds_t*)(size_t)dc, 4, 0, 0);
+    btile_prefetch(a, td, lane, lds);
+  };
+  int tile = __builtin_amdgcn_readfirstlane(next_tile(btile()));
+  if (tile < a.ntiles) fetch(tile);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the first tile is in LDS
+  while (tile < a.ntiles) {
+    const int p = td.p;
+    const long long lo = td.lo, hi = td.hi, first = td.first;
+    // an opaque offset: the context's reads are ordered by the explicit
+    // waits, and the compiler adds none of its own for the transfer into it
+    unsigned co = 0;
+    asm volatile("" : "+s"(co));
+    const double lmc = ctx[co], tb0c = ctx[co + 2], tb1c = ctx[co + 3];
+    const long long is1c = a.has_next ? __double_as_longlong(ctx[co + 1]) : -2;
+    double ginc[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) ginc[k] = ctx[co + 4 + k];
+    const long long i = first + lane - 1;
+    const bool inr = i >= lo && i < hi;
+    const bool owned = inr && lane >= 1 && lane <= TILE;
+    const bool hit = is1c >= first - 1 && is1c <= first + TILE;  // wave-uniform
+    double q[NQ], qb[NQ];
 #pragma unroll
     for (int r = 0; r < NQ; ++r) {
-      const double v = qb[r] + ib[r];
-      a.Qb_out[r * pitch + i] = v;
-      zero = zero && v == 0.0;
+      q[r] = buf[64 * r + lane];
+      qb[r] = inr ? buf[64 * (NQ + 1 + r) + lane] : 0.0;
     }
-    th0 += ib[NQ];
-    th1 += ib[NQ + 1];
-  }
-  const int z = __all(zero);
-  dts = wave_sum(dts);
-  th0 = wave_sum(th0);
-  th1 = wave_sum(th1);
-  if (lane == 0) {
-    a.zout[tile] = z;
-    a.dtp[tile] = dts;
-    a.thb[2 * tile] += th0;
-    a.thb[2 * tile + 1] += th1;
+    // the state's sound speeds, stored in row 14 of its slot by the forward step
+    const double cs = buf[64 * NQ + lane];
+    // every lane's reads are in registers before the next tile overwrites the buffer
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    const int tn = __builtin_amdgcn_readfirstlane(next_tile(tile + W));
+    if (tn < a.ntiles) fetch(tn);
+    double dts = 0.0, th0 = 0.0, th1 = 0.0;
+    // lmax term of step s+1 at its argmax cell
+    if (hit) {
+      double acc = 0.0;
+      for (int b = a.tfirst[p] + lane; b < a.tfirst[p + 1]; b += 64) acc += a.dtp_next[b];
+      acc = wave_sum(acc);
+      const double lm1 = lmax_of(a.lmax, (a.s + 1) * a.P + p);
+      const double dtdx1 = a.cfl * rcp(lm1);  // as chunk_step forms dt / dx
+      const double lbar = -acc * dtdx1 * rcp(lm1);
+      if (inr && i == is1c) {
+        const double* J = a.Jlam + ((a.s + 1) * a.P + p) * JL;
+#pragma unroll
+        for (int r = 0; r < NQ; ++r) qb[r] += lbar * J[r];
+        if (owned) {
+          th0 += lbar * J[NQ];
+          th1 += lbar * J[NQ + 1];
+        }
+      }
+    }
+    const bool interior = inr && i > lo && i < hi - 1;
+    const double dtdx = a.cfl * rcp(lmc);  // as chunk_step forms it
+    double ib[hes_gen::CELL_IN];
+    reverse_chunk(q, qb, interior, owned, cs, 0.5 * dtdx, ginc, ib, dts);
+    bool zero = true;
+    if (owned) {
+#pragma unroll
+      for (int r = 0; r < NQ; ++r) {
+        const double v = qb[r] + ib[r];
+        a.Qb_out[r * pitch + i] = v;
+        zero = zero && v == 0.0;
+      }
+      th0 += ib[NQ];
+      th1 += ib[NQ + 1];
+    }
+    const int z = __all(zero);
+    dts = wave_sum(dts);
+    th0 = wave_sum(th0);
+    th1 = wave_sum(th1);
+    if (lane == 0) {
+      a.zout[tile] = z;
+      a.dtp[tile] = dts;
+      a.thb[2 * tile] = tb0c + th0;
+      a.thb[2 * tile + 1] = tb1c + th1;
+    }
+    // The next tile is in LDS once at most the 14 stores of Qbar above are
+    // in flight: vector memory operations retire in issue order, and every
+    // tile owns at least one cell, so all 14 are issued.
+    __builtin_amdgcn_s_waitcnt(0x0F7E);  // vmcnt(14)
+    tile = tn;
   }
 }
 
@@ -522,6 +635,7 @@ __global__ __launch_bounds__(64) void k_badj_final(
 // ---------------------------------------------------------------------------
 struct hes_badjoint {
   int P = 0, ntiles = 0, npieces = 0;
+  int nbr = 0;  // waves of the reverse step (persistent)
   long long N = 0, pitch = 0, nsteps = 0, every = 0, nck = 0;
   int sparse = 1;
   int use_graph = 1;
@@ -720,7 +834,8 @@ hipError_t brecord_backward(hes_badjoint* h) {
       a.P = h->P;
       a.has_next = s + 1 < h->nsteps ? 1 : 0;
       a.sparse = h->sparse;
-      hipLaunchKernelGGL(k_badj_step, dim3(nt), dim3(64), 0, st, a);
+      a.ntiles = h->ntiles;
+      hipLaunchKernelGGL(k_badj_step, dim3(h->nbr), dim3(64), 0, st, a);
     }
     HIP_TRY_E(hipEventRecord(h->ev_done[buf], st));
   }
@@ -809,6 +924,17 @@ int hes_badjoint_create(hes_badjoint** out, int nproblems, const long long* ncel
   h->tfirst_h[nproblems] = (int)tdesc.size();
   h->ntiles = (int)tdesc.size();
   h->N = h->offs_h[nproblems];
+  if (h->N >= (1LL << 28)) { delete h; return fail(HES_ERR_ARG, "at most 2^28 - 1 cells in all"); }
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus < 1) {
+      delete h;
+      return fail(HES_ERR_HIP, "cannot read the device's compute-unit count");
+    }
+    h->nbr = std::min(h->ntiles, 4 * cus);  // one reverse wave per SIMD
+  }
   h->pitch = (h->N + 63) / 64 * 64 + 256;  // rows padded off a power-of-two stride, as the solver's
   const size_t Pt = (size_t)h->pitch;
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
