@@ -27,6 +27,25 @@ struct TileDesc {
   int p, pad;
 };
 
+// Reads through the constant address space, for data a kernel only reads:
+// wave-uniform addresses then load into scalar registers, and no wait for
+// them waits for the wave's vector memory operations.
+template <class T>
+__device__ __forceinline__ T cload(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)p;
+}
+__device__ __forceinline__ TileDesc tile_record(const TileDesc* tiles, int t) {
+  typedef const __attribute__((address_space(4))) TileDesc ctile_t;
+  ctile_t* r = (ctile_t*)tiles + t;
+  TileDesc d;
+  d.first = r->first;
+  d.lo = r->lo;
+  d.hi = r->hi;
+  d.p = r->p;
+  d.pad = 0;
+  return d;
+}
+
 struct BatchArgs {
   const double* __restrict__ Qin;
   double* __restrict__ Qout;
@@ -90,27 +109,45 @@ template <int NPH, int FLUX, int FAST, int PIN>
 __device__ __forceinline__ void batch_tile_step(const BatchArgs& a, int tile) {
   constexpr int NC = NComp<NPH>::value;
   const int lane = threadIdx.x & 63;
-  const TileDesc td = a.tiles[tile];
+  const TileDesc td = tile_record(a.tiles, tile);
   const int p = td.p;
   const long long lo = td.lo, hi = td.hi, first = td.first;
   const bool lead = first == lo;  // this tile does the problem's bookkeeping
-  // the problem's materials are read in place (wave-uniform loads); a local
-  // copy whose address is passed on would live in scratch memory
+  // Two phases: the problem's materials in scalar registers (a local copy,
+  // read through the constant address space).  One phase: read in place,
+  // which measured faster there.
+  MatDev mloc[NPH];
   const MatDev* m = a.mats + NPH * p;
-  const ChunkCtx ctx = {m,  a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, lo, hi, a.cfl[p], a.dx[p],
-                        0.0, NPH == 2 ? a.dpath : nullptr, a.lxf};
+  if constexpr (NPH == 2) {
+    typedef const __attribute__((address_space(4))) MatDev cmat_t;
+#pragma unroll
+    for (int k = 0; k < NPH; ++k) {
+      cmat_t* g = (cmat_t*)(m + k);
+      MatDev& d = mloc[k];
+      d.rho0 = g->rho0; d.irho0 = g->irho0; d.cv = g->cv; d.t0 = g->t0; d.b0sq = g->b0sq;
+      d.k0 = g->k0; d.alpha = g->alpha; d.beta = g->beta; d.gamma = g->gamma;
+      d.hk0a2 = g->hk0a2; d.k0_2a = g->k0_2a; d.k0_a = g->k0_a; d.cvt0 = g->cvt0;
+      d.icvt0 = g->icvt0; d.hb0sq = g->hb0sq; d.half_a = g->half_a; d.half_b = g->half_b;
+      d.half_g = g->half_g; d.ac_t = g->ac_t; d.ac_s = g->ac_s; d.ac_e = g->ac_e;
+      d.ac_w = g->ac_w; d.fast = g->fast; d.pad = 0;
+    }
+    m = mloc;
+  }
+  const ChunkCtx ctx = {m,  a.Qin, a.Qout, a.Cin, a.Cout, a.status, a.pitch, lo, hi, cload(a.cfl + p),
+                        cload(a.dx + p), 0.0, NPH == 2 ? a.dpath : nullptr, a.lxf};
   const long long i = first + lane - 1;
   double q[NC], cc[NPH];
   load_chunk<NPH>(ctx, i, q, cc);  // in flight while the problem's time is read
-  const double tcur = a.tring[2 * p + (a.step & 1)];
-  if (!(tcur < a.t_end[p])) {
+  // this launch writes the other time and CFL slots of the problem
+  const double tcur = cload(a.tring + 2 * p + (a.step & 1));
+  if (!(tcur < cload(a.t_end + p))) {
     // the problem has reached its end time: it does not step, and its state
     // stays in the buffer of its last step (plaunch), from where
     // batch_carry moves it when the host reads it or changes end times
     if (lead && lane == 0) a.tring[2 * p + ((a.step + 1) & 1)] = tcur;
     return;
   }
-  const double lmax = __longlong_as_double(*(const long long*)(a.slots + 3 * p + (a.step % 3)));
+  const double lmax = __longlong_as_double((long long)cload(a.slots + 3 * p + (a.step % 3)));
   // two phases: the path terms of the jump faces come from k_bpath2 / k_bpath2_hll
   double lam = chunk_step<NPH, FLUX, FAST, NPH == 2, PIN != 0>(ctx, i, q, cc, lmax);
   lam = wave_max(lam);
@@ -178,12 +215,13 @@ __global__ __launch_bounds__(MARK_BLK) void k_bmark2(BatchArgs a) {
   long long i = 0;
   int p = 0;
   if (tile < a.tile_end) {
-    const TileDesc td = a.tiles[tile];
+    const TileDesc td = tile_record(a.tiles, tile);
     p = td.p;
     i = td.first + lane;
     const long long ic = i < td.hi - 1 ? i : td.hi - 2;  // a face of the problem on every lane
     const bool jump_here = face_jump2(a.Qin, ic, a.pitch);
-    const bool live = a.tring[2 * p + (a.step & 1)] < a.t_end[p];  // the problem steps
+    // the problem steps (its times are written by the step kernel only)
+    const bool live = cload(a.tring + 2 * p + (a.step & 1)) < cload(a.t_end + p);
     jump = live && lane < TILE && i < td.hi - 1 && jump_here;
   }
   const unsigned long long pos = block_append<MARK_BLK>(jump, a.jcnt + a.jslot);
