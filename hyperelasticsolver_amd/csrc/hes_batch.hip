@@ -27,13 +27,6 @@ struct TileDesc {
   int p, pad;
 };
 
-// Reads through the constant address space, for data a kernel only reads:
-// wave-uniform addresses then load into scalar registers, and no wait for
-// them waits for the wave's vector memory operations.
-template <class T>
-__device__ __forceinline__ T cload(const T* p) {
-  return *(const __attribute__((address_space(4))) T*)p;
-}
 __device__ __forceinline__ TileDesc tile_record(const TileDesc* tiles, int t) {
   typedef const __attribute__((address_space(4))) TileDesc ctile_t;
   ctile_t* r = (ctile_t*)tiles + t;

@@ -15,6 +15,15 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// A read through the constant address space, for data that a kernel only
+// reads (written by earlier launches): at a wave-uniform address it loads
+// into scalar registers, and no wait for it waits for the wave's vector
+// memory operations.
+template <class T>
+__device__ __forceinline__ T cload(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)p;
+}
+
 // Streaming access to the state: each element is read once and written once
 // per step, so it bypasses cache allocation.
 #ifndef HES_NT_LOAD
