@@ -70,9 +70,12 @@ struct BatchArgs {
   int tpw, nwaves;
   // two phases: the half path integrals of the launch's jump faces
   // (k_bpath2, k_bpath2_hll; face i+1/2 at [r * pitch + i]) and the jump-face
-  // list they come from (k_bmark2): face index and problem per entry, counted
-  // in jcnt[jslot].  The slot alternates per launch; each scan clears the
-  // other one for the next launch.
+  // list they come from (k_bmark2): face index and problem per entry.  The
+  // list has JSEG segments of jcap entries, each with its own counter (the
+  // scan's blocks append to segment block % JSEG, so no one counter takes
+  // every block's atomic); segment k of slot s counts in
+  // jcnt[(s JSEG + k) JSTRIDE].  The slot alternates per launch; each scan
+  // clears the other one for the next launch.
   const double* dpath;
   unsigned long long* jcnt;
   long long* jlist;
@@ -80,6 +83,13 @@ struct BatchArgs {
   long long jcap;
   int jslot;
 };
+
+#ifndef HES_JSEG
+#define HES_JSEG 32
+#endif
+constexpr int JSEG = HES_JSEG;   // jump-list segments (at most 64)
+constexpr int JSTRIDE = 16;      // counters 128 bytes apart
+constexpr size_t JCNT_BYTES = 2 * JSEG * JSTRIDE * sizeof(unsigned long long);
 
 // This wave's index in the grid (wave-uniform).
 template <int BLK>
@@ -198,9 +208,13 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_batch_step(BatchArgs a) {
 // block 0 clears the other slot for the next launch.  One wave per tile, 16
 // per block, one atomic per block; the state and the problem's time are
 // loaded together.
-constexpr int MARK_BLK = 1024;
+#ifndef HES_BMARK_BLK
+#define HES_BMARK_BLK 1024
+#endif
+constexpr int MARK_BLK = HES_BMARK_BLK;
 __global__ __launch_bounds__(MARK_BLK) void k_bmark2(BatchArgs a) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.jcnt[a.jslot ^ 1] = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x < JSEG)
+    a.jcnt[((a.jslot ^ 1) * JSEG + threadIdx.x) * JSTRIDE] = 0ull;
   const int tile = __builtin_amdgcn_readfirstlane(
       a.tile_begin + (int)((blockIdx.x * MARK_BLK + threadIdx.x) >> 6));
   const int lane = threadIdx.x & 63;
@@ -217,36 +231,77 @@ __global__ __launch_bounds__(MARK_BLK) void k_bmark2(BatchArgs a) {
     const bool live = cload(a.tring + 2 * p + (a.step & 1)) < cload(a.t_end + p);
     jump = live && lane < TILE && i < td.hi - 1 && jump_here;
   }
-  const unsigned long long pos = block_append<MARK_BLK>(jump, a.jcnt + a.jslot);
+  const int seg = (int)(blockIdx.x % JSEG);
+  const unsigned long long pos =
+      block_append<MARK_BLK>(jump, a.jcnt + (a.jslot * JSEG + seg) * JSTRIDE);
   if (jump && pos < (unsigned long long)a.jcap) {
-    a.jlist[pos] = i;
-    a.jprob[pos] = p;
+    const long long e = seg * a.jcap + (long long)pos;
+    a.jlist[e] = i;
+    a.jprob[e] = p;
   }
+}
+
+// The jump-list segments as one sequence: pre[k] (k < JSEG) receives the
+// inclusive prefix of the segment counts (each capped at jcap), pre[k] for
+// k >= JSEG the total, which is returned.  Every lane of the wave runs it.
+__device__ __forceinline__ unsigned jump_prefix(const BatchArgs& a, unsigned* pre) {
+  const int lane = threadIdx.x & 63;
+  unsigned c = 0;
+  if (lane < JSEG) {
+    const unsigned long long n = a.jcnt[(a.jslot * JSEG + lane) * JSTRIDE];
+    c = (unsigned)min(n, (unsigned long long)a.jcap);
+  }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned u = __shfl_up(c, o);
+    if (lane >= o) c += u;
+  }
+  pre[lane] = c;
+  return __shfl(c, 63);
+}
+
+// Entry of the j-th listed face (j < total): a binary search of the prefix
+// table.  The table is read from LDS, not from other lanes, because lanes
+// whose loop has ended are inactive.
+__device__ __forceinline__ long long jump_entry(const BatchArgs& a, const unsigned* pre,
+                                                unsigned j) {
+  int k = 0;
+#pragma unroll
+  for (int step = 32; step >= 1; step >>= 1) {
+    if (step >= JSEG) continue;
+    if (pre[k + step - 1] <= j) k += step;
+  }
+  const unsigned base = k ? pre[k - 1] : 0u;
+  return k * a.jcap + (long long)(j - base);
 }
 
 // The path terms of the listed jump faces with each face's own materials,
 // HES_PATH_LANES lanes per face for Rusanov and HES_PATH_HLL_LANES for HLL (as
-// the one-problem k_path2 / k_path2_hll).
+// the one-problem k_path2 / k_path2_hll).  One wave per block.
 template <int FAST>
 __global__ __launch_bounds__(64) void k_bpath2(BatchArgs a) {
-  const unsigned long long cnt = min(a.jcnt[a.jslot], (unsigned long long)a.jcap);
+  __shared__ unsigned pre[64];
+  const unsigned cnt = jump_prefix(a, pre);
   constexpr int GL = HES_PATH_LANES;
-  const unsigned long long stride = (unsigned long long)gridDim.x * (64 / GL);
-  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) / GL; j < cnt;
-       j += stride)
-    face_path2<FAST>(a.mats + 2 * a.jprob[j], a.Qin, (double*)a.dpath, a.jlist[j], a.pitch,
+  const unsigned stride = gridDim.x * (64 / GL);
+  for (unsigned j = (blockIdx.x * 64 + threadIdx.x) / GL; j < cnt; j += stride) {
+    const long long e = jump_entry(a, pre, j);
+    face_path2<FAST>(a.mats + 2 * a.jprob[e], a.Qin, (double*)a.dpath, a.jlist[e], a.pitch,
                      threadIdx.x & (GL - 1));
+  }
 }
 
 template <int FAST>
 __global__ __launch_bounds__(64) void k_bpath2_hll(BatchArgs a) {
-  const unsigned long long cnt = min(a.jcnt[a.jslot], (unsigned long long)a.jcap);
+  __shared__ unsigned pre[64];
+  const unsigned cnt = jump_prefix(a, pre);
   constexpr int GL = HES_PATH_HLL_LANES;
-  const unsigned long long stride = (unsigned long long)gridDim.x * (64 / GL);
-  for (unsigned long long j = ((unsigned long long)blockIdx.x * 64 + threadIdx.x) / GL; j < cnt;
-       j += stride)
-    face_path2_hll<FAST>(a.mats + 2 * a.jprob[j], a.Qin, a.Cin, (double*)a.dpath, a.jlist[j],
+  const unsigned stride = gridDim.x * (64 / GL);
+  for (unsigned j = (blockIdx.x * 64 + threadIdx.x) / GL; j < cnt; j += stride) {
+    const long long e = jump_entry(a, pre, j);
+    face_path2_hll<FAST>(a.mats + 2 * a.jprob[e], a.Qin, a.Cin, (double*)a.dpath, a.jlist[e],
                          a.pitch, threadIdx.x & (GL - 1));
+  }
 }
 
 // Sound speeds and per-problem CFL maximum of the current state.
@@ -397,6 +452,7 @@ struct hes_batch {
   unsigned long long* jcnt;
   long long* jlist;
   int* jprob;
+  long long jcap;  // entries per list segment
   int jpar;  // count slot of the next launch
   long long step;
   hipStream_t stream;
@@ -437,7 +493,7 @@ BatchArgs batch_args(hes_batch* h, long long s, int tb, int te) {
   a.jcnt = h->jcnt;
   a.jlist = h->jlist;
   a.jprob = h->jprob;
-  a.jcap = h->N;
+  a.jcap = h->jcap;
   a.jslot = h->jpar;
   return a;
 }
@@ -549,7 +605,7 @@ int batch_after_set(hes_batch* h, const double* t0) {
   HIP_TRY(zero_on(h->stream, h->psteps, sizeof(int) * h->P));
   HIP_TRY(zero_on(h->stream, h->plaunch, sizeof(long long) * h->P));
   HIP_TRY(zero_on(h->stream, h->status, sizeof(int) * 4));
-  if (h->jcnt) HIP_TRY(zero_on(h->stream, h->jcnt, 2 * sizeof(unsigned long long)));
+  if (h->jcnt) HIP_TRY(zero_on(h->stream, h->jcnt, JCNT_BYTES));
   h->jpar = 0;
   h->step = 0;
   h->has_state = true;
@@ -644,10 +700,14 @@ int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const
   up((void**)&h->slots, nullptr, sizeof(unsigned long long) * 3 * nproblems);
   if (nphase == 2) {
     up((void**)&h->dpath, nullptr, 2 * qb);  // D-, D+ rows
-    up((void**)&h->jcnt, nullptr, 2 * sizeof(unsigned long long));
-    up((void**)&h->jlist, nullptr, sizeof(long long) * (size_t)h->N);
-    up((void**)&h->jprob, nullptr, sizeof(int) * (size_t)h->N);
-    if (e == hipSuccess) e = zero_on(h->stream, h->jcnt, 2 * sizeof(unsigned long long));
+    // a scan block lists at most MARK_BLK / 64 tiles' faces, and segment k
+    // takes the blocks k, k + JSEG, ...
+    const long long nblk = (h->ntiles + MARK_BLK / 64 - 1) / (MARK_BLK / 64);
+    h->jcap = (nblk + JSEG - 1) / JSEG * (MARK_BLK / 64) * TILE;
+    up((void**)&h->jcnt, nullptr, JCNT_BYTES);
+    up((void**)&h->jlist, nullptr, sizeof(long long) * (size_t)(JSEG * h->jcap));
+    up((void**)&h->jprob, nullptr, sizeof(int) * (size_t)(JSEG * h->jcap));
+    if (e == hipSuccess) e = zero_on(h->stream, h->jcnt, JCNT_BYTES);
   }
   up((void**)&h->psteps, nullptr, sizeof(int) * nproblems);
   up((void**)&h->plaunch, nullptr, sizeof(long long) * nproblems);
