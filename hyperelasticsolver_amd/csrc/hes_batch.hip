@@ -204,40 +204,102 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_batch_step(BatchArgs a) {
 
 // Two phases: the faces with a volume-fraction jump in the launch's tiles
 // (face i+1/2 for each owned cell i < hi - 1 of a problem that steps),
-// appended with their problems to jlist / jprob and counted in jcnt[jslot];
-// block 0 clears the other slot for the next launch.  One wave per tile, 16
-// per block, one atomic per block; the state and the problem's time are
-// loaded together.
+// appended with their problems to the jump list (segment block % JSEG of
+// slot jslot); block 0 clears the other slot for the next launch.  Each wave
+// scans MARK_TPW consecutive tiles, issuing all their loads before it tests
+// any, and a block appends its waves' faces with one atomic.
 #ifndef HES_BMARK_BLK
 #define HES_BMARK_BLK 1024
 #endif
+#ifndef HES_BMARK_TPW
+#define HES_BMARK_TPW 4
+#endif
 constexpr int MARK_BLK = HES_BMARK_BLK;
+constexpr int MARK_TPW = HES_BMARK_TPW;
+
+// Offset of this lane's first entry in a list that every wave of a block of
+// NT threads appends to, the lane contributing c entries: one atomic per
+// block on *count, the waves' shares in block order, the lanes' in lane
+// order.  Every thread of the block must call it.
+template <int NT>
+__device__ __forceinline__ unsigned long long block_append_n(unsigned c, unsigned long long* count) {
+  __shared__ unsigned long long share[NT / 64];
+  __shared__ unsigned long long base;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  unsigned incl = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned u = __shfl_up(incl, o);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) share[w] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long total = 0ull;
+#pragma unroll
+    for (int k = 0; k < NT / 64; ++k) {
+      const unsigned long long n = share[k];
+      share[k] = total;
+      total += n;
+    }
+    base = total ? atomicAdd(count, total) : 0ull;
+  }
+  __syncthreads();
+  return base + share[w] + (unsigned long long)(incl - c);
+}
+
 __global__ __launch_bounds__(MARK_BLK) void k_bmark2(BatchArgs a) {
   if (blockIdx.x == 0 && threadIdx.x < JSEG)
     a.jcnt[((a.jslot ^ 1) * JSEG + threadIdx.x) * JSTRIDE] = 0ull;
-  const int tile = __builtin_amdgcn_readfirstlane(
-      a.tile_begin + (int)((blockIdx.x * MARK_BLK + threadIdx.x) >> 6));
+  const int t0 = __builtin_amdgcn_readfirstlane(
+      a.tile_begin + (int)((blockIdx.x * MARK_BLK + threadIdx.x) >> 6) * MARK_TPW);
   const int lane = threadIdx.x & 63;
-  bool jump = false;
-  long long i = 0;
-  int p = 0;
-  if (tile < a.tile_end) {
-    const TileDesc td = tile_record(a.tiles, tile);
-    p = td.p;
-    i = td.first + lane;
-    const long long ic = i < td.hi - 1 ? i : td.hi - 2;  // a face of the problem on every lane
-    const bool jump_here = face_jump2(a.Qin, ic, a.pitch);
-    // the problem steps (its times are written by the step kernel only)
-    const bool live = cload(a.tring + 2 * p + (a.step & 1)) < cload(a.t_end + p);
-    jump = live && lane < TILE && i < td.hi - 1 && jump_here;
+  double v[MARK_TPW][4];
+  long long idx[MARK_TPW];
+  int prob[MARK_TPW];
+  bool ok[MARK_TPW];
+#pragma unroll
+  for (int k = 0; k < MARK_TPW; ++k) {
+    const int tile = t0 + k;
+    ok[k] = false;
+    idx[k] = 0;
+    prob[k] = 0;
+    if (tile < a.tile_end) {
+      const TileDesc td = tile_record(a.tiles, tile);
+      const long long i = td.first + lane;
+      const long long ic = i < td.hi - 1 ? i : td.hi - 2;  // a face of the problem on every lane
+      v[k][0] = a.Qin[ic];
+      v[k][1] = a.Qin[ic + 1];
+      v[k][2] = a.Qin[15 * a.pitch + ic];
+      v[k][3] = a.Qin[15 * a.pitch + ic + 1];
+      // the problem steps (its times are written by the step kernel only)
+      const bool live =
+          cload(a.tring + 2 * td.p + (a.step & 1)) < cload(a.t_end + td.p);
+      ok[k] = live && lane < TILE && i < td.hi - 1;
+      idx[k] = i;
+      prob[k] = td.p;
+    }
+  }
+  bool jump[MARK_TPW];
+  unsigned c = 0;
+#pragma unroll
+  for (int k = 0; k < MARK_TPW; ++k) {
+    // face_jump2's test
+    jump[k] = ok[k] && !(((v[k][1] - v[k][0]) == 0.0) & ((v[k][3] - v[k][2]) == 0.0));
+    c += jump[k] ? 1u : 0u;
   }
   const int seg = (int)(blockIdx.x % JSEG);
-  const unsigned long long pos =
-      block_append<MARK_BLK>(jump, a.jcnt + (a.jslot * JSEG + seg) * JSTRIDE);
-  if (jump && pos < (unsigned long long)a.jcap) {
-    const long long e = seg * a.jcap + (long long)pos;
-    a.jlist[e] = i;
-    a.jprob[e] = p;
+  unsigned long long pos = block_append_n<MARK_BLK>(c, a.jcnt + (a.jslot * JSEG + seg) * JSTRIDE);
+#pragma unroll
+  for (int k = 0; k < MARK_TPW; ++k) {
+    if (jump[k]) {
+      if (pos < (unsigned long long)a.jcap) {
+        const long long e = seg * a.jcap + (long long)pos;
+        a.jlist[e] = idx[k];
+        a.jprob[e] = prob[k];
+      }
+      ++pos;
+    }
   }
 }
 
@@ -543,8 +605,9 @@ void launch_batch(hes_batch* h, const BatchArgs& a) {
     // grid (as many waves as stay resident: two per SIMD for Rusanov, one
     // for HLL), then the step
     hipStream_t st = h->stream;
-    hipLaunchKernelGGL(k_bmark2, dim3((unsigned)((tiles + MARK_BLK / 64 - 1) / (MARK_BLK / 64))),
-                       dim3(MARK_BLK), 0, st, a);
+    constexpr int TPB = MARK_BLK / 64 * MARK_TPW;  // tiles per scan block
+    hipLaunchKernelGGL(k_bmark2, dim3((unsigned)((tiles + TPB - 1) / TPB)), dim3(MARK_BLK), 0, st,
+                       a);
     const long long nbf = ((long long)tiles * TILE + 254) / 256;
     const unsigned nbl = (unsigned)std::min<long long>(
         nbf, h->flux != HES_FLUX_HLL ? 2 * 1024 : 1024);
@@ -700,10 +763,11 @@ int hes_batch_create(hes_batch** out, int nphase, int flux, int nproblems, const
   up((void**)&h->slots, nullptr, sizeof(unsigned long long) * 3 * nproblems);
   if (nphase == 2) {
     up((void**)&h->dpath, nullptr, 2 * qb);  // D-, D+ rows
-    // a scan block lists at most MARK_BLK / 64 tiles' faces, and segment k
-    // takes the blocks k, k + JSEG, ...
-    const long long nblk = (h->ntiles + MARK_BLK / 64 - 1) / (MARK_BLK / 64);
-    h->jcap = (nblk + JSEG - 1) / JSEG * (MARK_BLK / 64) * TILE;
+    // a scan block lists at most TPB tiles' faces, and segment k takes the
+    // blocks k, k + JSEG, ...
+    constexpr int TPB = MARK_BLK / 64 * MARK_TPW;
+    const long long nblk = (h->ntiles + TPB - 1) / TPB;
+    h->jcap = (nblk + JSEG - 1) / JSEG * TPB * TILE;
     up((void**)&h->jcnt, nullptr, JCNT_BYTES);
     up((void**)&h->jlist, nullptr, sizeof(long long) * (size_t)(JSEG * h->jcap));
     up((void**)&h->jprob, nullptr, sizeof(int) * (size_t)(JSEG * h->jcap));
