@@ -350,7 +350,7 @@ __device__ __forceinline__ void tile_prefetch(const AdjStepArgs& a, long long w,
   }
 }
 
-// Persistent waves, one per SIMD (a wave holds 256 VGPRs + 78 AGPRs, so a
+// Persistent waves, one per SIMD (a wave holds 256 VGPRs + 76 AGPRs, so a
 // forward wave of the recompute fits beside it): wave k of the grid takes
 // tiles k, k + W, k + 2W, ...  While it evaluates one tile, the next one it
 // will evaluate streams into its LDS buffer, so only the first tile's loads
