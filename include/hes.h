@@ -165,7 +165,9 @@ int hes_batch_autotune(hes_batch* h, int steps, int* block, int* xcd, int* tiles
  *      by the backward pass.  Only alpha = 1, beta = 3, gamma = 2 materials. */
 typedef struct hes_adjoint hes_adjoint;
 
-/* material: the 8 Barton2009 constants; nsteps: steps of the run (no end time). */
+/* material: the 8 Barton2009 constants; nsteps: steps of the run (no end time);
+ * ncells below 2^28 (the reverse step addresses a row's bytes with 32-bit
+ * offsets), else HES_ERR_ARG. */
 int hes_adjoint_create(hes_adjoint** out, long long ncells, const double* material, double cfl,
                        double dx, long long nsteps, long long every);
 int hes_adjoint_destroy(hes_adjoint* h);
@@ -210,7 +212,8 @@ int hes_adjoint_info(hes_adjoint* h, long long* device_bytes, long long* checkpo
 typedef struct hes_badjoint hes_badjoint;
 
 /* materials: 8 Barton2009 constants per problem (8 x nproblems); dx: one per
- * problem, or NULL for 1 / ncells[p]; nsteps: steps of every problem. */
+ * problem, or NULL for 1 / ncells[p]; nsteps: steps of every problem; fewer
+ * than 2^28 cells in all, else HES_ERR_ARG. */
 int hes_badjoint_create(hes_badjoint** out, int nproblems, const long long* ncells,
                         const double* materials, double cfl, const double* dx, long long nsteps,
                         long long every);

@@ -89,6 +89,7 @@ def test_batch_create_rejects_a_bad_material_and_dx(L):
     (100, MAT_GEN, 0.6, 10, 5, "exponents"),
     (100, MAT, -0.6, 10, 5, "positive"),
     (100, MAT, 0.6, 1 << 40, 5, "too many steps"),
+    (1 << 28, MAT, 0.6, 10, 5, "2^28"),
 ])
 def test_adjoint_create_rejects(L, ncells, mat, cfl, nsteps, every, words):
     m = np.ascontiguousarray(mat)
@@ -105,6 +106,7 @@ def test_adjoint_create_rejects(L, ncells, mat, cfl, nsteps, every, words):
     ([100, 200], [MAT, MAT], None, 0, 5, "at least one step"),
     ([100, 200], [MAT, MAT], None, 10, 0, "interval"),
     ([100, 200], [MAT, MAT], None, (1 << 31) - 1, 5, "too many steps"),
+    ([1 << 27, 1 << 27], [MAT, MAT], None, 10, 5, "2^28"),
 ])
 def test_batched_adjoint_create_rejects(L, ncells, mats, dx, nsteps, every, words):
     nc, pnc = _ll(ncells)
