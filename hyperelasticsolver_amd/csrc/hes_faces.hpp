@@ -45,16 +45,20 @@ __device__ __forceinline__ void phase_thermo(const MatDev* mat, const double* q,
 }
 
 // Flux F(Q) of a cell and the wave-speed bounds of its eigenvalues
-// [min_p(u1 - c_p), max_p(u1 + c_p)] from the stored sound speeds cc[p].
-template <int NPH, int FAST = -1>
+// [min_p(u1 - c_p), max_p(u1 + c_p)] from the stored sound speeds cc[p]
+// (NOC, one phase: from the cell's state instead).
+template <int NPH, int FAST = -1, bool NOC = false>
 __device__ __forceinline__ void cell_flux(const MatDev* mat, const double* q, const double* cc,
                                           double* f, double& smin, double& smax, int& bad) {
   if (NPH == 1) {
     Thermo t;
     thermo<true, FAST>(mat[0], 1.0, q, t);
     phase_flux(mat[0], 1.0, t, q + 5, f);
-    smin = t.u[0] - cc[0];
-    smax = t.u[0] + cc[0];
+    // NOC: the sound speed from this thermodynamic state (the same bits as
+    // the stored one, which the step that wrote the state computed so)
+    const double c0 = NOC ? sound_speed(mat[0], t) : cc[0];
+    smin = t.u[0] - c0;
+    smax = t.u[0] + c0;
     bad = t.bad;
   } else {
     bad = 0;

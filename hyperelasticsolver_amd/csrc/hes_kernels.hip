@@ -38,6 +38,8 @@ constexpr int BLOCK = HES_STEP_BLOCK;  // threads per block of the step kernel
 #ifndef HES_STEP2_WAVES_PER_EU
 #define HES_STEP2_WAVES_PER_EU 2
 #endif
+// single phase: the pinned schedule (chunk_step's PIN) from this many cells
+constexpr long long PIN_MIN_CELLS = 1LL << 21;
 
 // ---------------------------------------------------------------------------
 // Step kernel
@@ -139,7 +141,9 @@ __attribute__((amdgpu_waves_per_eu(NPH == 1 ? HES_STEP_WAVES_PER_EU : HES_STEP2_
   const double lmax = step_lmax(a);
   prio_compute();
   constexpr bool PB = NPH == 2;  // path integrals from k_path2 / k_path2_hll
-  double lam = chunk_step<NPH, FLUX, FAST, PB, PIN != 0, SRC != 0>(ctx, i, q, cc, lmax);
+  // the pinned single-phase Rusanov step keeps no sound speeds (see chunk_step)
+  constexpr bool NOC = NPH == 1 && FLUX == FLUX_RUSANOV && PIN != 0;
+  double lam = chunk_step<NPH, FLUX, FAST, PB, PIN != 0, SRC != 0, NOC>(ctx, i, q, cc, lmax);
   step_epilogue(a, lam, tcur, lmax);
 }
 
@@ -522,7 +526,7 @@ void launch_step_t(const StepArgs& a, long long nchunks, bool fast, hipStream_t 
   }
   if constexpr (NPH == 1) {
     // single phase from 2^21 cells: the pinned schedule (chunk_step)
-    if (a.n >= (1LL << 21)) {
+    if (a.n >= PIN_MIN_CELLS) {
       if (fast)
         hipLaunchKernelGGL((k_step<NPH, FLUX, 1, 1>), dim3((unsigned)nblocks), dim3(BLOCK), 0, s, a);
       else
@@ -824,6 +828,15 @@ int hes_solver_get_state(hes_solver* h, double* Q) {
 
 int hes_solver_get_speeds(hes_solver* h, double* c) {
   if (!h || !c) return fail(HES_ERR_ARG, "null argument");
+  if (h->nph == 1 && h->flux != HES_FLUX_HLL && h->n >= PIN_MIN_CELLS) {
+    // the pinned Rusanov steps keep no sound speeds: form them for the
+    // current state (the CFL maximum goes to the spare fourth slot)
+    const long long nb = (h->n + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(k_speeds<1>, dim3((unsigned)nb), dim3(BLOCK), 0, h->stream,
+                       h->Q[h->step & 1], h->C[h->step & 1], h->n, h->pitch, h->mat[0], h->mat[1],
+                       h->slots + 3, h->status);
+    HIP_TRY(hipGetLastError());
+  }
   for (int p = 0; p < h->nph; ++p)
     HIP_TRY(copy_on(h->stream, c + p * h->n, h->C[h->step & 1] + p * h->pitch,
                     sizeof(double) * h->n, hipMemcpyDeviceToHost));

@@ -67,7 +67,9 @@ int hes_solver_set_piecewise(hes_solver* h, int nstates, const double* states,
                              const double* interfaces, double t0);
 /* Download the current state (Julia layout). */
 int hes_solver_get_state(hes_solver* h, double* Q_host);
-/* Per-phase sound speed max_k sqrt|eig_k(Omega)| of every cell, (nphase, ncells). */
+/* Per-phase sound speed max_k sqrt|eig_k(Omega)| of every cell, (nphase, ncells)
+ * (single-phase Rusanov from 2^21 cells, whose steps keep none: formed for
+ * the current state on the call). */
 int hes_solver_get_speeds(hes_solver* h, double* c_host);
 /* Time and number of steps taken since the state was set. */
 int hes_solver_time(hes_solver* h, double* t, long long* steps);

@@ -225,6 +225,35 @@ def test_parity_from_two_to_the_21_cells(ora, flux):
     assert np.array_equal(Qg[:, ns:], Q0[:, ns:])
 
 
+def test_pinned_rusanov_speeds_are_formed_on_request(ora):
+    """The pinned single-phase Rusanov step (from 2^21 cells) keeps no sound
+    speeds; get_speeds forms them for the current state.  After some steps
+    they are the speeds of the state reached (bit for bit those a new solver
+    forms from it), not those of the initial state."""
+    from hyperelasticsolver_amd.EquationsOfState import Barton2009
+    from hyperelasticsolver_amd.solver import Solver
+    ns, steps, nx = 4000, 20, (1 << 21) + 37
+    print(f"[pinned-speeds] nx={nx} steps={steps}", flush=True)
+    mats, S, _, ifc = _problem(ora, 1, ns, 6, seed=505)
+    Q0 = np.empty((S.shape[0], nx), order="F")
+    Q0[:, :ns] = S
+    Q0[:, ns:] = S[:, -1:]
+    eos = Barton2009(_rho0=mats[0], _c0=mats[1], _cv=mats[2], _t0=mats[3], _b0=mats[4],
+                     _alpha=mats[5], _beta=mats[6], _gamma=mats[7])
+    with Solver(eos, nx, flux="rusanov", cfl=0.6) as s:
+        s.set_state(Q0)
+        c0 = s.get_speeds()
+        s.run(steps=steps)
+        c1 = s.get_speeds()
+        Q1 = s.get_state()
+    with Solver(eos, nx, flux="rusanov", cfl=0.6) as s2:
+        s2.set_state(Q1)
+        c_ref = s2.get_speeds()
+    assert np.array_equal(c1, c_ref)
+    moved = np.any(Q1 != Q0, axis=0)
+    assert moved.sum() > 100 and np.any(c1[0, moved] != c0[0, moved])
+
+
 @pytest.mark.parametrize("name,flux", [("config1_rusanov", "rusanov"), ("config1_hll", "hll")])
 def test_golden_config1_full_run(name, flux):
     """1000-cell Riemann problem run to t_end = 0.06 (701 steps) against the
