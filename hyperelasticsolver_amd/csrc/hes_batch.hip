@@ -152,7 +152,11 @@ __device__ __forceinline__ void batch_tile_step(const BatchArgs& a, int tile) {
   }
   const double lmax = __longlong_as_double((long long)cload(a.slots + 3 * p + (a.step % 3)));
   // two phases: the path terms of the jump faces come from k_bpath2 / k_bpath2_hll
-  double lam = chunk_step<NPH, FLUX, FAST, NPH == 2, PIN != 0>(ctx, i, q, cc, lmax);
+  // one phase, Rusanov: every launch shape forms the input sound speeds from
+  // the state and stores none (see chunk_step's NOC), so switching shapes or
+  // launch modes never reads a stale stored speed
+  double lam = chunk_step<NPH, FLUX, FAST, NPH == 2, PIN != 0, false,
+                          NPH == 1 && FLUX == FLUX_RUSANOV>(ctx, i, q, cc, lmax);
   lam = wave_max(lam);
   if (lane == 0) {
     unsigned long long* slot = a.slots + 3 * p + ((a.step + 1) % 3);

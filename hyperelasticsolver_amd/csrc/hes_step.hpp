@@ -318,8 +318,9 @@ __device__ __forceinline__ void relax_pressure2(const MatDev* mat, double* q, do
 // rather than read (cc is not used), and the new state's is not stored: 16 B
 // less traffic per cell for about 170 more VALU instructions per wave.  The
 // values are the same bits either way.  It pays for the memory-bound pinned
-// Rusanov step (2^22 cells: 185.6-186.1 -> 182.1-182.8 us) and costs the
-// VALU-bound HLL one (197.6-197.9 -> 206.8-207.0 us), same box.
+// Rusanov step (2^22 cells: 185.6-186.1 -> 182.1-182.8 us) and the batched
+// Rusanov step (4096 problems, 16.2M cells: 823.5-824.5 -> 785.3-786.8 us)
+// and costs the VALU-bound HLL one (197.6-197.9 -> 206.8-207.0 us), same box.
 template <int NPH, int FLUX, int FAST, bool PATHBUF = false, bool PIN = false, bool SRC = false,
           bool NOC = false>
 __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, const double* q,
