@@ -89,6 +89,41 @@ def test_batch_matches_one_problem_adjoints():
         assert np.abs(gmb - gma).max() <= 1e-9 * scale, (p, gmb, gma)
 
 
+def test_batch_of_more_tiles_than_reverse_waves():
+    """More tiles than the reverse step has persistent waves (4 per CU): the
+    waves take several tiles each, every later one prefetched into LDS while
+    the previous one is evaluated.  Three problems' gradients agree with the
+    one-problem adjoint (which gets one wave per tile here); skipping and
+    graph replay change no bit."""
+    from hyperelasticsolver_amd.adjoint import Adjoint
+    ncells = [2000 + 37 * k for k in range(48)]
+    ntiles = sum((n + 61) // 62 for n in ncells)
+    assert ntiles > 4 * 256, ntiles
+    print(f"[badjoint] {len(ncells)} problems, {ntiles} tiles", flush=True)
+    pieces = [2] * len(ncells)
+    steps, every = 40, 9
+    eos, prims, ifcs, rho_t, u_t = _batch_problem(ncells, pieces, steps, seed=47)
+    L, Q, grads = _batch_run(eos, ncells, prims, ifcs, rho_t, u_t, steps, every)
+    L2, Q2, grads2 = _batch_run(eos, ncells, prims, ifcs, rho_t, u_t, steps, every,
+                                sparse=False, graph=False)
+    assert np.array_equal(L, L2) and np.array_equal(Q, Q2)
+    for (g, gm), (g2, gm2) in zip(grads, grads2):
+        assert np.array_equal(g, g2) and np.array_equal(gm, gm2)
+    off = np.concatenate([[0], np.cumsum(ncells)])
+    for p in (0, 23, 47):
+        sl = slice(off[p], off[p + 1])
+        with Adjoint(eos[p], ncells[p], steps, every=every) as a:
+            a.set_initial(prims[p], list(ifcs[p]))
+            a.set_target(rho_t[sl], u_t[sl])
+            La = a.forward()
+            ga, gma = a.backward()
+        assert L[p] == pytest.approx(La, rel=1e-12, abs=1e-300)
+        gb, gmb = grads[p]
+        scale = max(np.abs(ga).max(), np.abs(gma).max())
+        assert np.abs(gb - ga).max() <= 1e-9 * scale, (p, np.abs(gb - ga).max(), scale)
+        assert np.abs(gmb - gma).max() <= 1e-9 * scale, (p, gmb, gma)
+
+
 def test_batch_with_its_own_dx_per_problem():
     """Per-problem dx (here half and twice 1 / n_p): each problem still evolves
     and differentiates as the one-problem adjoint with that dx."""
