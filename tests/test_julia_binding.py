@@ -1,5 +1,6 @@
-"""The Julia binding a maintainer adds (the `HesGPU` module in INTEGRATION.md)
-against the C-ABI it calls: every `ccall` names a function include/hes.h
+"""The Julia binding a maintainer adds (the `HesGPU` module of the
+integration notes at the repository root, NOTES below) against the C-ABI it
+calls: every `ccall` names a function include/hes.h
 declares, passes as many arguments as it lists types, and lists the types of
 the C prototype in order (Julia is absent from this image, so this is the
 check that the stub would link and pass arguments correctly).  The module
@@ -11,6 +12,7 @@ import re
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NOTES = os.path.join(ROOT, "INTEGRATION.md")
 
 
 def _strip_c_comments(text):
@@ -69,9 +71,9 @@ def _split_top(text):
 
 
 def julia_block():
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    text = open(NOTES).read()
     m = re.search(r"```julia\n(module HesGPU.*?)```", text, re.S)
-    assert m, "no HesGPU module in INTEGRATION.md"
+    assert m, "no HesGPU module in the integration notes"
     return m.group(1)
 
 
@@ -114,7 +116,7 @@ _RET = {"Cint": "int", "Cstring": "const char *"}
 
 
 def all_julia_blocks():
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    text = open(NOTES).read()
     return "\n".join(re.findall(r"```julia\n(.*?)```", text, re.S))
 
 
