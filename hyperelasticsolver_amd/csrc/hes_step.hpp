@@ -476,9 +476,12 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
       // lane 0's update read the zeros, and neither is stored
       const double J = lane_down_z(fr) - fr;
       const double dq = lane_down_z(q[r]) - q[r];
-      const double dm = hc.cl * J + hc.clr * dq;
-      const double dp = hc.cr * J - hc.clr * dq;
-      upd[r] = q[r] - dtdx * (dm + lane_up_z(dp));  // conservative part is zero (NumFluxes.jl:82)
+      // D- + D+ = J at every face (cl + cr = 1), so D-(i+1/2) + D+(i-1/2) =
+      // H(i+1/2) - H(i-1/2) with H = F_i + D-(i+1/2), the HLL flux: two
+      // fused products per face instead of four, and a uniform state still
+      // gives H = F_i exactly
+      const double h = fma(hc.cl, J, fma(hc.clr, dq, fr));
+      upd[r] = q[r] - dtdx * (h - lane_up_z(h));  // conservative part is zero (NumFluxes.jl:82)
       if (PHASED || PIN) pin(upd[r]);
     };
     if (PHASED) {
