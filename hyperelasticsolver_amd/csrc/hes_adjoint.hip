@@ -30,6 +30,9 @@
 // Qbar_s: the one or two waves holding the argmax cell reduce the block
 // partials of step s and add lmaxbar_s * dlam/d(Q, b0, c0) of that cell
 // (computed once per segment).
+// the secular-function eigenvalue iteration throughout (see sym3_extreme);
+// set before any include
+#define HES_EIG_SECULAR 1
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>

@@ -8,6 +8,10 @@
 #include "hes_adjoint_gen.hpp"
 #include "hes_step.hpp"
 
+// A batch and a single problem must evolve bit for bit alike, so both
+// adjoint sources take the same eigenvalue iteration.
+static_assert(HES_EIG_SECULAR == 1, "the adjoint's sources define HES_EIG_SECULAR 1");
+
 namespace hes_adj {
 
 using namespace hes;

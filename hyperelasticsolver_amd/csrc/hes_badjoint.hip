@@ -21,6 +21,9 @@
 // reverse steps follow hes_adjoint.hip: every `every`-th state is kept, and a
 // backward segment's states are recomputed on a second stream into one of two
 // segment buffers while the reverse steps of the previous segment run.
+// the secular-function eigenvalue iteration throughout (see sym3_extreme);
+// set before any include
+#define HES_EIG_SECULAR 1
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>

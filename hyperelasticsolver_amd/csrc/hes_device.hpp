@@ -376,64 +376,121 @@ __device__ __forceinline__ double rcp1(double x) {
   return fma(fma(-x, r, 1.0), r, r);
 }
 
+// HES_EIG_SECULAR = 1 leaves out sym3_extreme's shifted-cubic path, so that
+// every state takes the secular-function iteration.  The adjoint's sources set
+// it: their launches (2^18 cells, a wave or two per SIMD) are bound by the
+// latency of one wave's dependent chain, which the secular iteration keeps
+// shorter (two steps of one reciprocal each, against a fixed-point step and two
+// Newton steps), while the solver's and the batch's steps are bound by
+// instruction throughput, where the cubic path issues about a third fewer.
+#ifndef HES_EIG_SECULAR
+#define HES_EIG_SECULAR 0
+#endif
+
 // Eigenvalue of largest modulus of a symmetric 3x3 given as O11 O12 O13 O22
 // O23 O33.  HyperelasticityMPh.jl:263 takes sqrt(|eig|) of all three; only the
 // largest modulus is ever consumed (main.jl:210, NumFluxes.jl:90-91).
 //
-// Write O = [a w^T; w D] with w = (O12, O13) and D the trailing 2x2 block.
-// When a exceeds lDu = max(O22, O33) + |O23|, an upper bound of D's
-// eigenvalues, the top eigenvalue is the root above a of the secular function
-//   g(l) = l - a - w^T (l - D)^-1 w,
-// which increases (g' >= 1) and is concave there, with g(a) <= 0: Newton from
-// l = a rises monotonically onto the root.  Its curvature is of order
-// |w|^2 / (a - lDu)^3, so for the acoustic tensor (a longitudinal entry well
-// above the shear block) two or three steps reach full precision.  A step
-// from l leaves an error of at most |g''(l)| / (2 g'(l)) step^2 <=
-// (p^2 + q^2) / det^2 / (l - lDu) step^2 (p, q, det as below), which the loop
-// tests with no further evaluation.  Otherwise (and for the rare state that
-// does not settle in 24 steps) Newton on the characteristic cubic from an
-// upper bound finds the top eigenvalue.  A bottom eigenvalue of larger
-// modulus is possible only below the Gershgorin bound lb, and then found by
-// Newton from below.
+// Write O = [a w^T; w D] with w = (O12, O13), D the trailing 2x2 block and
+// M = a I - D.  Since adj(M + mu I) = adj(M) + mu I for a 2x2, the top
+// eigenvalue is a + mu with mu the root in [0, inf) of the shifted cubic
+//   h(mu) = det((a + mu) I - O) = mu^3 + T mu^2 + s mu - n0,
+//   T = tr M, s = det M - |w|^2, n0 = w^T adj(M) w,
+// whose coefficients carry no cancellation when a stands apart from D (the
+// acoustic tensor's longitudinal entry above its shear block).  With T > 0
+// and s > 0, h is increasing and convex on [0, inf), so that root is the
+// only one there, and n0 / s (Newton from 0) bounds it from above.  One
+// fixed-point step mu = n0 / (mu^2 + T mu + s) from that bound lands below
+// the root, and Newton steps from there stay in [0, n0 / s], on which
+// h'' / 2 <= 3 n0 / s + T and h' >= s.  A step st from mu, taken with slope
+// h'(mu), then leaves an error of at most (3 n0 / s + T) h'(mu) st^2 / s^2,
+// which is tested against 2^-52 (a + mu).  States with a 5% random distortion
+// take two Newton steps, strongly sheared ones three or four.  A state with T <= 0 or s <= 0
+// (or that does not settle in ten steps) goes to the secular function
+//   g(l) = l - a - w^T (l - D)^-1 w
+// by Newton from l = a when a exceeds lDu = max(O22, O33) + |O23|, an upper
+// bound of D's eigenvalues (g increases and is concave there, with g(a) <= 0,
+// so the iterates rise monotonically onto the root; a step from l leaves an
+// error of at most (p^2 + q^2) / det^2 / (l - lDu) step^2, p, q, det as
+// below), and otherwise to Newton on the characteristic cubic from an upper
+// bound.  A bottom eigenvalue of larger modulus needs tr O < top (it is
+// tr O - top - (middle eigenvalue) >= tr O - 2 top), and then lies below the
+// Gershgorin bound lb, where Newton from below finds it.
 __device__ __forceinline__ double sym3_extreme(const double O[6]) {
   const double a = O[0], b = O[3], c = O[5], d = O[1], e = O[2], f = O[4];
-  const double r1 = fabs(d) + fabs(e), r2 = fabs(d) + fabs(f), r3 = fabs(e) + fabs(f);
-  const double lb = fmin(a - r1, fmin(b - r2, c - r3));
-  const double lDu = fmax(b, c) + fabs(f);
   double top = a;
   bool done = false;
-  if (a > lDu) {
-    const double ff = f * f, fd = f * d, fe = f * e;
+  if constexpr (!HES_EIG_SECULAR) {
+    const double amb = a - b, amc = a - c, T = amb + amc;
+    const double d2 = d * d, e2 = e * e;
+    const double s = fma(amb, amc, -(f * f)) - (d2 + e2);
+    const double n0 = fma(amc, d2, fma(amb, e2, (2.0 * f) * d * e));
+    if (T > 0.0 && s > 0.0) {
+      const double is = rcp1(s);
+      const double mu0 = n0 * is;                     // >= the root
+      const double hb = fma(3.0, mu0, T) * (is * is);
+      double mu = n0 * rcp1(fma(mu0 + T, mu0, s));    // <= the root
+      double st = 0.0, hp = 1.0;
+      auto newton = [&]() {
+        const double h = fma(fma(mu + T, mu, s), mu, -n0);
+        hp = fma(fma(3.0, mu, 2.0 * T), mu, s);
+        st = -h * rcp1(hp);
+        mu += st;
+      };
+      newton();
+      newton();
+      top = a + mu;
+      done = hb * hp * (st * st) <= 0x1p-52 * top;  // false for NaN and overflow
 #pragma unroll 1
-    for (int it = 0; it < 24; ++it) {
-      const double x = top - b, y = top - c;
-      const double det = x * y - ff;       // det(l - D) > 0
-      const double p = y * d + fe, q = x * e + fd;  // adj(l - D) w
-      const double h = (top - a) * det - (d * p + e * q);  // g(l) det
-      const double pq2 = p * p + q * q, det2 = det * det;
-      const double st = -h * det * rcp1(det2 + pq2);  // -g / g'
-      const double lo = top;
-      top += st;
-      if (!(pq2 * (st * st) > 0x1p-52 * lo * (lo - lDu) * det2)) {
-        done = true;
-        break;
+      for (int it = 0; it < 8 && !done; ++it) {
+        newton();
+        top = a + mu;
+        done = hb * hp * (st * st) <= 0x1p-52 * top;
       }
     }
   }
   if (!done) {
-    const double c2 = a + b + c;
-    const double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
-    const double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
-    const double ub = fmax(a + r1, fmax(b + r2, c + r3));  // Gershgorin
-    top = cubic_top_root(c2, c1, c0, ub, lb);
+    const double lDu = fmax(b, c) + fabs(f);
+    top = a;
+    if (a > lDu) {
+      const double ff = f * f, fd = f * d, fe = f * e;
+#pragma unroll 1
+      for (int it = 0; it < 24; ++it) {
+        const double x = top - b, y = top - c;
+        const double det = x * y - ff;       // det(l - D) > 0
+        const double p = y * d + fe, q = x * e + fd;  // adj(l - D) w
+        const double h = (top - a) * det - (d * p + e * q);  // g(l) det
+        const double pq2 = p * p + q * q, det2 = det * det;
+        const double st = -h * det * rcp1(det2 + pq2);  // -g / g'
+        const double lo = top;
+        top += st;
+        if (!(pq2 * (st * st) > 0x1p-52 * lo * (lo - lDu) * det2)) {
+          done = true;
+          break;
+        }
+      }
+    }
+    if (!done) {
+      const double r1 = fabs(d) + fabs(e), r2 = fabs(d) + fabs(f), r3 = fabs(e) + fabs(f);
+      const double c2 = a + b + c;
+      const double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
+      const double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
+      const double ub = fmax(a + r1, fmax(b + r2, c + r3));  // Gershgorin
+      const double lb = fmin(a - r1, fmin(b - r2, c - r3));
+      top = cubic_top_root(c2, c1, c0, ub, lb);
+    }
   }
   double mu = top;
-  if (lb < -fabs(top)) {
-    const double c2 = a + b + c;
-    const double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
-    const double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
-    const double bot = cubic_bottom_root(c2, c1, c0, lb, fmax(a + r1, fmax(b + r2, c + r3)));
-    if (fabs(bot) > fabs(top)) mu = bot;
+  if (!(a + b + c >= top)) {
+    const double r1 = fabs(d) + fabs(e), r2 = fabs(d) + fabs(f), r3 = fabs(e) + fabs(f);
+    const double lb = fmin(a - r1, fmin(b - r2, c - r3));
+    if (lb < -fabs(top)) {
+      const double c2 = a + b + c;
+      const double c1 = a * b - d * d + a * c - e * e + b * c - f * f;
+      const double c0 = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
+      const double bot = cubic_bottom_root(c2, c1, c0, lb, fmax(a + r1, fmax(b + r2, c + r3)));
+      if (fabs(bot) > fabs(top)) mu = bot;
+    }
   }
   return mu;
 }
