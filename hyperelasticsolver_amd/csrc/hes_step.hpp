@@ -339,24 +339,33 @@ __device__ __forceinline__ double chunk_step(const ChunkCtx& a, long long i, con
   // (from u1 and the stored sound speeds), then each phase's flux is formed
   // and consumed by the update in turn, so only one phase's flux is live
   constexpr bool PHASED = NPH == 2 && PATHBUF;
+  // The mid state Q_m = (Q_i + Q_i+1) / 2 enters as the neighbours' sum
+  // S = 2 Q_m (0.5 S is exact) with volume fraction 2 a_m (one phase: 2):
+  // thermo's density takes det(S) / (2 a_m)^3, so every power of two scales
+  // out exactly and the thermodynamic state, sound speed and range are those
+  // of Q_m to the bit, without the 14 or 15 halvings per phase.
   double mn_m = 0.0, mx_m = 0.0;
   if (HLL_STREAM && PHASED) {
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-      double qm[15];
+      double qs[15];
 #pragma unroll
-      for (int r = 0; r < 15; ++r) qm[r] = 0.5 * (q[15 * p + r] + lane_down(q[15 * p + r]));
+      for (int r = 0; r < 15; ++r) qs[r] = q[15 * p + r] + lane_down(q[15 * p + r]);
       Thermo t;
-      thermo<false, FAST>(a.mat[p], qm[0], qm + 1, t);
+      thermo<false, FAST>(a.mat[p], qs[0], qs + 1, t);
       const double c = sound_speed(a.mat[p], t);
       mn_m = p == 0 ? t.u[0] - c : fmin(mn_m, t.u[0] - c);
       mx_m = p == 0 ? t.u[0] + c : fmax(mx_m, t.u[0] + c);
     }
-  } else if (HLL_STREAM) {
-    double qm[NC];
+  } else if (HLL_STREAM && NPH == 1) {
+    double qs[NC];
 #pragma unroll
-    for (int r = 0; r < NC; ++r) qm[r] = 0.5 * (q[r] + lane_down(q[r]));
-    state_speed_range<NPH, FAST>(a.mat, qm, mn_m, mx_m);
+    for (int r = 0; r < NC; ++r) qs[r] = q[r] + lane_down(q[r]);
+    Thermo t;
+    thermo<false, FAST>(a.mat[0], 2.0, qs, t);
+    const double c = sound_speed(a.mat[0], t);
+    mn_m = t.u[0] - c;
+    mx_m = t.u[0] + c;
   }
 
   double f[NC], smin, smax;
