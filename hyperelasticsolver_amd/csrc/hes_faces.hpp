@@ -95,21 +95,6 @@ __device__ __forceinline__ double cell_speeds(const MatDev* mat, const double* q
   return lam;
 }
 
-// Eigenvalue range of a state from scratch (HLL mid state, NumFluxes.jl:90-91).
-template <int NPH, int FAST = -1>
-__device__ __forceinline__ void state_speed_range(const MatDev* mat, const double* q, double& smin,
-                                                  double& smax) {
-#pragma unroll
-  for (int p = 0; p < NPH; ++p) {
-    Thermo t;
-    phase_thermo<NPH, FAST>(mat, q, p, t);
-    double c = sound_speed(mat[NPH == 1 ? 0 : p], t);
-    double lo = t.u[0] - c, hi = t.u[0] + c;
-    smin = p == 0 ? lo : fmin(smin, lo);
-    smax = p == 0 ? hi : fmax(smax, hi);
-  }
-}
-
 // Column 1 of the non-conservative matrix of each phase block at state q
 // (HyperelasticityMPh.jl:178-231 with omega = 0, k = (1/2, 1/2), beta = 0).
 // Rows 1, 3:5 and 6 are common to both blocks; rows 7:15 are per phase.
@@ -243,15 +228,27 @@ __device__ __forceinline__ HllCoef hll_coef(double mn_m, double mx_m, double smi
   return c;
 }
 
-// Eigenvalue range of the mid state Q_m = (Q_l + Q_r) / 2 of a face.
+// Eigenvalue range of the mid state Q_m = (Q_l + Q_r) / 2 of a face, formed
+// from the sum S = Q_l + Q_r = 2 Q_m with volume fraction 2 a_m (one phase:
+// 2): thermo's density takes det(S) / (2 a_m)^3, and every power of two
+// scales out exactly, so the state and its range are Q_m's to the bit.
 template <int NPH, int FAST = -1>
 __device__ __forceinline__ void mid_speed_range(const MatDev* mat, const double* ql,
                                                 const double* qr, double& mn_m, double& mx_m) {
   constexpr int NC = NComp<NPH>::value;
-  double qm[NC];
+  double qs[NC];
 #pragma unroll
-  for (int r = 0; r < NC; ++r) qm[r] = 0.5 * (ql[r] + qr[r]);
-  state_speed_range<NPH, FAST>(mat, qm, mn_m, mx_m);
+  for (int r = 0; r < NC; ++r) qs[r] = ql[r] + qr[r];
+#pragma unroll
+  for (int p = 0; p < NPH; ++p) {
+    Thermo t;
+    if (NPH == 1) thermo<false, FAST>(mat[0], 2.0, qs, t);
+    else thermo<false, FAST>(mat[p], qs[15 * p], qs + 15 * p + 1, t);
+    const double c = sound_speed(mat[NPH == 1 ? 0 : p], t);
+    const double lo = t.u[0] - c, hi = t.u[0] + c;
+    mn_m = p == 0 ? lo : fmin(mn_m, lo);
+    mx_m = p == 0 ? hi : fmax(mx_m, hi);
+  }
 }
 
 // HLL face (NumFluxes.jl:70-132): D- and D+ (the conservative part is zero).

@@ -793,11 +793,12 @@ __device__ __forceinline__ void face_path2_hll(const MatDev* mat, const double* 
   // eigenvalue range of the mid state (Q_l + Q_r) / 2, phase g on lane g
   double mlo = 0.0, mhi = 0.0;
   if (g < 2) {
-    double qm[15];
+    // from the neighbours' sum 2 Q_m, as the step forms it (the same bits)
+    double qs[15];
 #pragma unroll
-    for (int r = 0; r < 15; ++r) qm[r] = 0.5 * ((g == 0 ? q[r] : q[15 + r]) + (g == 0 ? qn[r] : qn[15 + r]));
+    for (int r = 0; r < 15; ++r) qs[r] = (g == 0 ? q[r] : q[15 + r]) + (g == 0 ? qn[r] : qn[15 + r]);
     Thermo t;
-    thermo<false, FAST>(mat[g], qm[0], qm + 1, t);
+    thermo<false, FAST>(mat[g], qs[0], qs + 1, t);
     const double c = sound_speed(mat[g], t);
     mlo = t.u[0] - c;
     mhi = t.u[0] + c;
