@@ -378,11 +378,12 @@ __device__ __forceinline__ double rcp1(double x) {
 
 // HES_EIG_SECULAR = 1 leaves out sym3_extreme's shifted-cubic path, so that
 // every state takes the secular-function iteration.  The adjoint's sources set
-// it: their launches (2^18 cells, a wave or two per SIMD) are bound by the
-// latency of one wave's dependent chain, which the secular iteration keeps
-// shorter (two steps of one reciprocal each, against a fixed-point step and two
-// Newton steps), while the solver's and the batch's steps are bound by
-// instruction throughput, where the cubic path issues about a third fewer.
+// it: their launches (a 2^18-cell problem runs a wave or two per SIMD) are
+// bound by the latency of one wave's dependent chain, which the secular
+// iteration keeps shorter (two steps of one reciprocal each, against a
+// fixed-point step and two Newton steps), while the solver's and the batch's
+// steps are bound by instruction throughput, where the cubic path issues
+// about a third fewer.
 #ifndef HES_EIG_SECULAR
 #define HES_EIG_SECULAR 0
 #endif
@@ -405,8 +406,9 @@ __device__ __forceinline__ double rcp1(double x) {
 // h'' / 2 <= 3 n0 / s + T and h' >= s.  A step st from mu, taken with slope
 // h'(mu), then leaves an error of at most (3 n0 / s + T) h'(mu) st^2 / s^2,
 // which is tested against 2^-52 (a + mu).  States with a 5% random distortion
-// take two Newton steps, strongly sheared ones three or four.  A state with T <= 0 or s <= 0
-// (or that does not settle in ten steps) goes to the secular function
+// take two Newton steps, strongly sheared ones three or four.  A state with
+// T <= 0 or s <= 0 (or that does not settle in ten steps) goes to the secular
+// function
 //   g(l) = l - a - w^T (l - D)^-1 w
 // by Newton from l = a when a exceeds lDu = max(O22, O33) + |O23|, an upper
 // bound of D's eigenvalues (g increases and is concave there, with g(a) <= 0,
@@ -414,8 +416,8 @@ __device__ __forceinline__ double rcp1(double x) {
 // error of at most (p^2 + q^2) / det^2 / (l - lDu) step^2, p, q, det as
 // below), and otherwise to Newton on the characteristic cubic from an upper
 // bound.  A bottom eigenvalue of larger modulus needs tr O < top (it is
-// tr O - top - (middle eigenvalue) >= tr O - 2 top), and then lies below the
-// Gershgorin bound lb, where Newton from below finds it.
+// tr O - top - (middle eigenvalue) >= tr O - 2 top) and a Gershgorin lower
+// bound lb < -|top|; Newton from lb upwards then finds it.
 __device__ __forceinline__ double sym3_extreme(const double O[6]) {
   const double a = O[0], b = O[3], c = O[5], d = O[1], e = O[2], f = O[4];
   double top = a;
