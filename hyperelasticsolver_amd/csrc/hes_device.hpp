@@ -81,6 +81,18 @@ __device__ __forceinline__ double sqrt_pos(double x) {
   return fma(fma(-g, g, x), h, g);
 }
 
+// Cube root of a positive normal v whose single-precision value is normal
+// (about 1e-37 to 1e38): an estimate from the hardware's single-precision
+// log2 and exp2 (a few 2^-23 relative), then one Halley step
+// y (y^3 + 2 v) / (2 y^3 + v), which cubes the relative error, so the
+// result is within a few ulps.
+__device__ __forceinline__ double cbrt_pos(double v) {
+  const float e = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)v) * (1.0f / 3.0f));
+  const double y = (double)e;
+  const double y3 = y * y * y;
+  return y * fma(2.0, v, y3) * rcp(fma(2.0, y3, v));
+}
+
 __device__ __forceinline__ double det3cm(const double* a) {
   // column-major a(i,j) = a[i + 3j]
   return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[3] * (a[1] * a[8] - a[7] * a[2]) +

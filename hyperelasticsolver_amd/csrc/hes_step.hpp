@@ -217,7 +217,7 @@ __device__ __forceinline__ double scaled_pressure(const MatDev& m, const PhaseSc
   if (FAST == 1 || (FAST == -1 && m.fast)) {
     const double ix2 = ix * ix;
     fa = ix;
-    fb = ix2 * ix2 * cbrt(ix);
+    fb = ix2 * ix2 * cbrt_pos(ix);
     fg = ix2;
   } else {
     const double lx = log(x);
@@ -293,7 +293,7 @@ __device__ __forceinline__ void relax_pressure2(const MatDev* mat, double* q, do
   }
   const double d = d1;
   const double a0n = a[0] + d, a1n = a[1] - d;
-  const double s0 = cbrt(a0n * rcp(a[0])), s1 = cbrt(a1n * rcp(a[1]));
+  const double s0 = cbrt_pos(a0n * rcp(a[0])), s1 = cbrt_pos(a1n * rcp(a[1]));
   q[0] = a0n;
   q[15] = a1n;
   q[5] -= pI * d;
