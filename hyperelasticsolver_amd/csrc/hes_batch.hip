@@ -342,8 +342,17 @@ __device__ __forceinline__ long long jump_entry(const BatchArgs& a, const unsign
 }
 
 // The path terms of the listed jump faces with each face's own materials,
-// HES_PATH_LANES lanes per face for Rusanov and HES_PATH_HLL_LANES for HLL (as
-// the one-problem k_path2 / k_path2_hll).  One wave per block.
+// HES_PATH_LANES lanes per face for Rusanov (as the one-problem k_path2) and
+// HES_BPATH_HLL_LANES for HLL.  A batch lists many jump faces (every
+// problem's interfaces and their spread), so the HLL form takes two lanes per
+// face, 32 faces per wave, against the one-problem k_path2_hll's four: each
+// lane then evaluates more quadrature nodes in series and spills more (184
+// against 84 B per lane), but the grid covers the list in half the rounds
+// (512 problems, steps 0 .. 20: 307.5-308.1 -> 300.0-305.8 us per step, same
+// box).  One wave per block.
+#ifndef HES_BPATH_HLL_LANES
+#define HES_BPATH_HLL_LANES 2
+#endif
 template <int FAST>
 __global__ __launch_bounds__(64) void k_bpath2(BatchArgs a) {
   __shared__ unsigned pre[64];
@@ -361,12 +370,12 @@ template <int FAST>
 __global__ __launch_bounds__(64) void k_bpath2_hll(BatchArgs a) {
   __shared__ unsigned pre[64];
   const unsigned cnt = jump_prefix(a, pre);
-  constexpr int GL = HES_PATH_HLL_LANES;
+  constexpr int GL = HES_BPATH_HLL_LANES;
   const unsigned stride = gridDim.x * (64 / GL);
   for (unsigned j = (blockIdx.x * 64 + threadIdx.x) / GL; j < cnt; j += stride) {
     const long long e = jump_entry(a, pre, j);
-    face_path2_hll<FAST>(a.mats + 2 * a.jprob[e], a.Qin, a.Cin, (double*)a.dpath, a.jlist[e],
-                         a.pitch, threadIdx.x & (GL - 1));
+    face_path2_hll<FAST, GL>(a.mats + 2 * a.jprob[e], a.Qin, a.Cin, (double*)a.dpath,
+                             a.jlist[e], a.pitch, threadIdx.x & (GL - 1));
   }
 }
 
